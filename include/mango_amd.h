@@ -1,0 +1,246 @@
+/* SPDX-License-Identifier: MIT
+ *
+ * C ABI of the MI355X (gfx950) American-option PDE library.
+ *
+ * Two groups of entry points live here:
+ *
+ *  1. The reference's existing pricing ABI, with identical names, struct
+ *     layouts and error conventions (mango-option src/ffi/mango_c_api.h).
+ *     Layouts are frozen by the static_asserts at the end of this file, the
+ *     same offsets the reference asserts (src/ffi/mango_c_api.h:195-214).
+ *
+ *  2. Batch, chain and adjoint entry points. The reference has no batch C
+ *     entry; these mirror BatchAmericanOptionSolver::solve_batch
+ *     (src/option/american_option_batch.hpp:171-175) with the C ABI's
+ *     conventions: caller-allocated output slots, a per-slot status, a failed
+ *     count and a nullable MangoError (src/ffi/mango_c_api.cpp:23-31).
+ *
+ * Every compute entry point runs on the GPU. There is no CPU fallback: when
+ * no gfx950 device is usable the call returns MANGO_ERR_SOLVER with a message.
+ */
+#ifndef MANGO_AMD_H
+#define MANGO_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status / enums: src/ffi/mango_c_api.h:16-26 ---------------------- */
+typedef int32_t MangoStatus;
+#define MANGO_OK 0
+#define MANGO_ERR_VALIDATION 1
+#define MANGO_ERR_ARBITRAGE 2
+#define MANGO_ERR_NO_CONVERGENCE 3
+#define MANGO_ERR_BRACKETING 4
+#define MANGO_ERR_SOLVER 5
+
+typedef int32_t MangoOptionType;
+#define MANGO_CALL 0
+#define MANGO_PUT 1
+
+/* ---- value types: src/ffi/mango_c_api.h:28-44 -------------------------- */
+typedef struct { int32_t code; char message[256]; } MangoError;
+typedef struct { double calendar_time; double amount; } MangoDividend;
+typedef struct { double tenor; double log_discount; } MangoTenorPoint;
+
+typedef struct {
+  double spot;
+  double strike;
+  double maturity;
+  double dividend_yield;
+  double volatility;
+  double rate_const;                    /* used iff n_tenor_points == 0 */
+  const MangoTenorPoint* tenor_points;
+  uint64_t n_tenor_points;
+  const MangoDividend* dividends;       /* may be null when n_dividends == 0 */
+  uint64_t n_dividends;
+  MangoOptionType option_type;
+} MangoPricingParams;
+
+/* ---- single option: replaces src/ffi/mango_c_api.h:141-150 ------------- */
+typedef struct MangoAmericanResult MangoAmericanResult; /* opaque */
+
+/* Auto grid (estimate_pde_grid with default GridAccuracyParams), one solve
+ * on the GPU, value/delta/gamma/theta computed eagerly. Replaces
+ * mango_price_american (src/ffi/mango_c_api.h:141-143,
+ * src/ffi/mango_c_api.cpp:309-361). */
+MangoStatus mango_price_american(const MangoPricingParams* params,
+                                 MangoAmericanResult** out_result,
+                                 MangoError* out_err);
+/* Getters return NaN on null (src/ffi/mango_c_api.cpp:363-366). */
+double mango_american_value(const MangoAmericanResult* r);
+double mango_american_delta(const MangoAmericanResult* r);
+double mango_american_gamma(const MangoAmericanResult* r);
+double mango_american_theta(const MangoAmericanResult* r);
+/* Spline interpolation of the stored final solution at any spot
+ * (src/ffi/mango_c_api.h:146-147; AmericanOptionResult::value_at,
+ * src/option/american_option_result.cpp:29-49). */
+MangoStatus mango_american_value_at(const MangoAmericanResult* r, double spot,
+                                    double* out, MangoError* out_err);
+void mango_american_result_free(MangoAmericanResult* r);
+
+/* ---- grid description (additions) --------------------------------------
+ * GridAccuracyParams (src/option/grid_spec_types.hpp:23-53). */
+typedef struct {
+  double n_sigma;
+  double alpha;
+  double tol;
+  double c_t;
+  uint64_t min_spatial_points;
+  uint64_t max_spatial_points;
+  uint64_t max_time_steps;
+} MangoGridAccuracy;
+
+/* MultiSinhCluster (src/pde/core/grid.hpp:24-28). */
+typedef struct { double center_x; double alpha; double weight; } MangoSinhCluster;
+
+#define MANGO_GRID_UNIFORM 0
+#define MANGO_GRID_LOG_SPACED 1
+#define MANGO_GRID_SINH 2
+#define MANGO_GRID_MULTI_SINH 3
+#define MANGO_MAX_CLUSTERS 4
+
+/* GridSpec (src/pde/core/grid.hpp:44-270). Clusters are taken as given
+ * (already merged) by the solver; mango_grid_spec_multi_sinh merges them
+ * exactly like GridSpec::multi_sinh_spaced. */
+typedef struct {
+  int32_t type;
+  int32_t n_clusters;
+  double x_min;
+  double x_max;
+  uint64_t n_points;
+  double concentration;
+  MangoSinhCluster clusters[MANGO_MAX_CLUSTERS];
+} MangoGridSpec;
+
+/* What PDEGridSpec (src/option/grid_spec_types.hpp:55-61) resolves to.
+ *  ACCURACY            : GridAccuracyParams -> estimate_pde_grid per option.
+ *  EXPLICIT            : PDEGridConfig{spec, n_time}.
+ *  ACCURACY_FIXED_TIME : spatial spec of estimate_pde_grid per option, paired
+ *                        with n_time as a PDEGridConfig (how a caller pins
+ *                        "auto space x N steps" in the reference).
+ * min_time_steps raises the step count of the ACCURACY mode (0 = off). */
+#define MANGO_GRID_MODE_ACCURACY 0
+#define MANGO_GRID_MODE_EXPLICIT 1
+#define MANGO_GRID_MODE_ACCURACY_FIXED_TIME 2
+
+typedef struct {
+  int32_t mode;
+  int32_t reserved;
+  MangoGridAccuracy accuracy;
+  MangoGridSpec spec;
+  uint64_t n_time;
+  uint64_t min_time_steps;
+} MangoGridConfig;
+
+/* Fill a MangoGridConfig with GridAccuracyParams{} defaults, mode ACCURACY. */
+void mango_grid_config_default(MangoGridConfig* cfg);
+
+/* GridSpec::multi_sinh_spaced with auto_merge (src/pde/core/grid.hpp:111-165,
+ * 191-235). Returns MANGO_ERR_VALIDATION on invalid input. */
+MangoStatus mango_grid_spec_multi_sinh(double x_min, double x_max, uint64_t n_points,
+                                       const MangoSinhCluster* clusters, uint64_t n_clusters,
+                                       MangoGridSpec* out, MangoError* err);
+
+/* estimate_pde_grid (src/option/grid_spec_types.cpp:37-118) and
+ * estimate_batch_pde_grid (src/option/grid_spec_types.cpp:120-177) on the
+ * host; accuracy may be null (defaults). */
+MangoStatus mango_estimate_pde_grid(const MangoPricingParams* params,
+                                    const MangoGridAccuracy* accuracy,
+                                    MangoGridSpec* out_spec, uint64_t* out_n_time,
+                                    MangoError* err);
+MangoStatus mango_estimate_batch_pde_grid(const MangoPricingParams* params, uint64_t n,
+                                          const MangoGridAccuracy* accuracy,
+                                          MangoGridSpec* out_spec, uint64_t* out_n_time,
+                                          MangoError* err);
+
+/* GridSpec::generate (src/pde/core/grid.hpp:364-498) on the host, so callers
+ * can query the x-grid a batch solve used. out_x has n_points slots. */
+MangoStatus mango_grid_generate(const MangoGridSpec* spec, double* out_x, MangoError* err);
+
+/* ---- batch entry points (additions) ------------------------------------ */
+typedef struct { double value; double delta; double gamma; double theta; } MangoAmericanGreeks;
+
+/* BatchAmericanOptionSolver::solve_batch(params, use_shared_grid, nullptr,
+ * custom_grid) with routing to the normalized chain path
+ * (src/option/american_option_batch.cpp:272-302). grid == NULL means no
+ * custom grid (auto estimation with default accuracy); a grid with mode
+ * ACCURACY sets the solver's GridAccuracyParams. out and out_status have n
+ * slots; out_status[i] is MANGO_OK or a MangoStatus; failed slots hold NaN.
+ * Host buffers; the call blocks until results are on the host. */
+MangoStatus mango_price_american_batch(const MangoPricingParams* params, uint64_t n,
+                                       int32_t use_shared_grid,
+                                       const MangoGridConfig* grid,
+                                       MangoAmericanGreeks* out, int32_t* out_status,
+                                       uint64_t* out_failed, MangoError* err);
+
+/* Same, with the solver's set_use_normalized switch exposed
+ * (src/option/american_option_batch.hpp:137-140). */
+MangoStatus mango_price_american_batch_ex(const MangoPricingParams* params, uint64_t n,
+                                          int32_t use_shared_grid, int32_t use_normalized,
+                                          const MangoGridConfig* grid,
+                                          MangoAmericanGreeks* out, int32_t* out_status,
+                                          uint64_t* out_failed, MangoError* err);
+
+/* Full-state variant: also returns the x-grid, the final solution u and the
+ * previous-step solution u_prev of every option (what AmericanOptionResult
+ * holds through its Grid, src/option/american_option_result.hpp:46-53).
+ * Arrays are [n][stride] with stride >= the largest n_space of the batch;
+ * out_n_space[i] and out_dt[i] receive the per-option grid size and time step. */
+MangoStatus mango_price_american_batch_state(const MangoPricingParams* params, uint64_t n,
+                                             const MangoGridConfig* grid, uint64_t stride,
+                                             double* out_x, double* out_u, double* out_u_prev,
+                                             uint64_t* out_n_space, double* out_dt,
+                                             MangoAmericanGreeks* out, int32_t* out_status,
+                                             uint64_t* out_failed, MangoError* err);
+
+/* ---- device-resident batch (inputs already in HBM) ----------------------
+ * Compact per-option record for device buffers; constant rate, no discrete
+ * dividends. */
+typedef struct {
+  double spot;
+  double strike;
+  double maturity;
+  double volatility;
+  double rate;
+  double dividend_yield;
+  int32_t option_type;
+  int32_t reserved;
+} MangoDeviceOption;
+
+/* Regular batch of per-option solves, one option per lane group, on the
+ * given hipStream_t (NULL = default stream). d_opts, d_out and d_status are
+ * device pointers. grid must have mode EXPLICIT or ACCURACY_FIXED_TIME with
+ * min_spatial_points == max_spatial_points (the grid size is then known on
+ * the host), or mode ACCURACY (per-option sizes are binned on the device).
+ * Asynchronous: returns after the launches are queued. */
+MangoStatus mango_price_american_batch_device(const MangoDeviceOption* d_opts, uint64_t n,
+                                              const MangoGridConfig* grid,
+                                              MangoAmericanGreeks* d_out, int32_t* d_status,
+                                              void* stream, MangoError* err);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#ifdef __cplusplus
+static_assert(sizeof(MangoPricingParams) == 88, "MangoPricingParams size");
+static_assert(offsetof(MangoPricingParams, rate_const) == 40, "rate_const off");
+static_assert(offsetof(MangoPricingParams, tenor_points) == 48, "tenor_points off");
+static_assert(offsetof(MangoPricingParams, n_dividends) == 72, "n_dividends off");
+static_assert(offsetof(MangoPricingParams, option_type) == 80, "option_type off");
+static_assert(sizeof(MangoDividend) == 16, "MangoDividend size");
+static_assert(sizeof(MangoTenorPoint) == 16, "MangoTenorPoint size");
+static_assert(sizeof(MangoError) == 260, "MangoError size");
+static_assert(offsetof(MangoError, message) == 4, "message off");
+static_assert(sizeof(MangoGridAccuracy) == 56, "MangoGridAccuracy size");
+static_assert(sizeof(MangoGridSpec) == 136, "MangoGridSpec size");
+static_assert(sizeof(MangoGridConfig) == 216, "MangoGridConfig size");
+static_assert(sizeof(MangoAmericanGreeks) == 32, "MangoAmericanGreeks size");
+static_assert(sizeof(MangoDeviceOption) == 56, "MangoDeviceOption size");
+#endif
+
+#endif /* MANGO_AMD_H */

@@ -1,0 +1,808 @@
+// SPDX-License-Identifier: MIT
+//
+// gfx950 kernels for the batched American-option PDE solve.
+//
+// Layout: a group of K lanes (K | 64, inside one wavefront) owns one option.
+// Lane j holds grid nodes [j*M, j*M+M) in registers for the whole time march:
+// u, u_prev, the obstacle psi, the analytic Jacobian bands of L and the
+// non-uniform stencil coefficients. Nothing is re-read from HBM inside the
+// march; HBM sees only the job record in and four Greeks (optionally the
+// final state) out.
+//
+// Each implicit stage A u = rhs, u >= psi (the projected Thomas / Brennan-
+// Schwartz solve of the reference, src/math/thomas_solver.hpp:321-420) runs as
+//   1. c'  : Riccati recurrence c'_i = up_i / (d_i - l_i c'_{i-1}), obtained as
+//            a cross-lane scan of 2x2 transfer matrices, then the exact local
+//            recurrence in each lane. c' and 1/denom depend only on the bands,
+//            i.e. on (stage weight, lock pattern), so they are cached per stage
+//            kind and recomputed only when the deep-ITM lock pattern changes.
+//   2. d'  : affine recurrence d'_i = (rhs_i - l_i d'_{i-1}) / denom_i: cross-
+//            lane scan of affine maps, then the exact local recurrence.
+//   3. u   : projected back substitution u_i = max(d'_i - c'_i u_{i+1}, psi_i):
+//            suffix scan of clamped-affine maps v -> clamp(a v + b, lo, hi)
+//            (closed under composition), then the exact local recurrence.
+// Within a lane every value is produced by the reference's own expression
+// (same fma placement); only the value entering a lane's block comes from a
+// scan, i.e. is reassociated (differences at the 1e-16 relative level).
+#include "american_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace mango_amd {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr double kBig = 1e300;
+constexpr double kSingularTol = 1e-15;   // ThomasConfig::singularity_tol
+constexpr double kDeepItmFraction = 0.95; // pde_solver.hpp:635
+constexpr double kExerciseTol = 1e-8;     // pde_solver.hpp:636
+
+// std::max / std::min semantics (first argument returned on ties and NaN).
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double clampd(double y, double lo, double hi) { return smin(smax(y, lo), hi); }
+
+template <int K>
+__device__ __forceinline__ double shup(double v, int d) { return __shfl_up(v, static_cast<unsigned>(d), K); }
+template <int K>
+__device__ __forceinline__ double shdn(double v, int d) { return __shfl_down(v, static_cast<unsigned>(d), K); }
+template <int K>
+__device__ __forceinline__ double group_max(double v) {
+#pragma unroll
+    for (int o = K / 2; o > 0; o >>= 1) v = smax(v, __shfl_xor(v, o, K));
+    return v;
+}
+template <int K>
+__device__ __forceinline__ double group_min(double v) {
+#pragma unroll
+    for (int o = K / 2; o > 0; o >>= 1) v = smin(v, __shfl_xor(v, o, K));
+    return v;
+}
+template <int K>
+__device__ __forceinline__ bool group_any(bool p) {
+    int v = p ? 1 : 0;
+#pragma unroll
+    for (int o = K / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, K);
+    return v != 0;
+}
+
+// Exact power-of-two rescale of a 2x2 matrix (ratios are what matter).
+__device__ __forceinline__ void renorm(double& a, double& b, double& c, double& d) {
+    const double m = fmax(fmax(fabs(a), fabs(b)), fmax(fabs(c), fabs(d)));
+    if (m > 0.0 && m < kBig) {
+        int e;
+        (void)frexp(m, &e);
+        a = ldexp(a, -e); b = ldexp(b, -e); c = ldexp(c, -e); d = ldexp(d, -e);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Serial pieces run by one lane on LDS rows (epilogue): natural cubic spline
+// (src/math/cubic_spline_solver.hpp:74-370) and the at-spot Greeks of
+// AmericanOptionResult (src/option/american_option_result.cpp:25-165).
+// ---------------------------------------------------------------------------
+// c[0..n-1] of the natural spline through (x, y); cp is scratch of n.
+__device__ bool spline_second(const double* x, const double* y, int n, double* cp, double* c) {
+    c[0] = 0.0;
+    c[n - 1] = 0.0;
+    if (n <= 2) return true;
+    const int m = n - 2;
+    // row i <-> knot k = i + 1; c[k] holds d'_i then the solution
+    auto h = [&](int k) { return x[k + 1] - x[k]; };
+    auto diag = [&](int i) { return 2.0 * (h(i) + h(i + 1)); };
+    auto rhs = [&](int i) {
+        const int k = i + 1;
+        const double slope_right = (y[k + 1] - y[k]) / h(k);
+        const double slope_left = (y[k] - y[k - 1]) / h(k - 1);
+        return 3.0 * (slope_right - slope_left);
+    };
+    if (m == 1) {
+        const double d0 = diag(0);
+        if (fabs(d0) < kSingularTol) return false;
+        c[1] = rhs(0) / d0;
+        return true;
+    }
+    {
+        const double d0 = diag(0);
+        if (fabs(d0) < kSingularTol) return false;
+        cp[1] = h(1) / d0;  // upper[0] = h[1]
+        c[1] = rhs(0) / d0;
+    }
+    for (int i = 1; i < m - 1; ++i) {
+        const double lo = h(i);
+        const double denom = fma(-lo, cp[i], diag(i));
+        if (fabs(denom) < kSingularTol) return false;
+        const double inv = 1.0 / denom;
+        cp[i + 1] = h(i + 1) * inv;
+        c[i + 1] = fma(-lo, c[i], rhs(i)) * inv;
+    }
+    {
+        const int i = m - 1;
+        const double lo = h(i);
+        const double denom = fma(-lo, cp[i], diag(i));
+        if (fabs(denom) < kSingularTol) return false;
+        c[i + 1] = fma(-lo, c[i], rhs(i)) / denom;
+    }
+    for (int i = m - 1; i > 0; --i) c[i] = fma(-cp[i], c[i + 1], c[i]);
+    return true;
+}
+
+__device__ int find_interval(const double* x, int n, double xe) {
+    if (xe <= x[0]) return 0;
+    if (xe >= x[n - 1]) return n - 2;
+    int lo = 0, hi = n;  // lower_bound: first idx with x[idx] >= xe
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (x[mid] < xe) lo = mid + 1; else hi = mid;
+    }
+    int idx = lo;
+    if (idx > 0 && (idx == n || x[idx] > xe)) --idx;
+    return idx < n - 2 ? idx : n - 2;
+}
+
+__device__ void spline_coeffs(const double* x, const double* y, const double* c, int i,
+                              double& a, double& b, double& ci, double& d) {
+    const double h = x[i + 1] - x[i];
+    a = y[i];
+    b = (y[i + 1] - y[i]) / h - h * (2.0 * c[i] + c[i + 1]) / 3.0;
+    ci = c[i];
+    d = (c[i + 1] - c[i]) / (3.0 * h);
+}
+
+__device__ double spline_eval(const double* x, const double* y, const double* c, int n, double xe) {
+    const int i = find_interval(x, n, xe);
+    double a, b, ci, d;
+    spline_coeffs(x, y, c, i, a, b, ci, d);
+    const double dx = xe - x[i];
+    return a + b * dx + ci * (dx * dx) + d * (dx * dx * dx);
+}
+
+__device__ double spline_deriv(const double* x, const double* y, const double* c, int n, double xe) {
+    const int i = find_interval(x, n, xe);
+    double a, b, ci, d;
+    spline_coeffs(x, y, c, i, a, b, ci, d);
+    const double dx = xe - x[i];
+    return b + 2.0 * ci * dx + 3.0 * d * (dx * dx);
+}
+
+__device__ bool grid_uniform(const double* x, int n) {
+    const double expected = (x[n - 1] - x[0]) / static_cast<double>(n - 1);
+    const double tol = 100.0 * 2.220446049250313e-16;
+    for (int i = 1; i < n; ++i)
+        if (fabs((x[i] - x[i - 1]) - expected) > tol) return false;
+    return true;
+}
+
+// CenteredDifference at node i (1 <= i <= n-2) (centered_difference.hpp:26-99).
+__device__ void stencil_at(const double* x, const double* u, int n, bool uniform, int i,
+                           double& d1, double& d2) {
+    if (uniform) {
+        const double dx = (x[n - 1] - x[0]) / static_cast<double>(n - 1);
+        const double dx_inv = 1.0 / dx;
+        const double dx2_inv = dx_inv * dx_inv;
+        const double half_dx_inv = dx_inv * 0.5;
+        d2 = fma(u[i + 1] + u[i - 1], dx2_inv, -2.0 * u[i] * dx2_inv);
+        d1 = (u[i + 1] - u[i - 1]) * half_dx_inv;
+    } else {
+        const double dl = x[i] - x[i - 1];
+        const double dr = x[i + 1] - x[i];
+        const double dc = 0.5 * (dl + dr);
+        const double dxl_inv = 1.0 / dl, dxr_inv = 1.0 / dr, dxc_inv = 1.0 / dc;
+        const double wl = dr / (dl + dr), wr = dl / (dl + dr);
+        const double fwd = (u[i + 1] - u[i]) * dxr_inv;
+        const double bwd = (u[i] - u[i - 1]) * dxl_inv;
+        d2 = (fwd - bwd) * dxc_inv;
+        const double t1 = wl * (u[i] - u[i - 1]) * dxl_inv;
+        const double t2 = wr * (u[i + 1] - u[i]) * dxr_inv;
+        d1 = t1 + t2;
+    }
+}
+
+// value/delta/gamma at spot from (x, u, c_u); theta also needs (u_prev, c_up).
+__device__ Greeks greeks_at(const double* x, const double* u, const double* cu, const double* up,
+                            const double* cup, int n, bool uniform, double S, double K, double dt) {
+    Greeks g;
+    const double xs = log(S / K);
+    const double v_spline = spline_eval(x, u, cu, n, xs);
+    if (xs <= x[0]) g.value = u[0] * K;
+    else if (xs >= x[n - 1]) g.value = u[n - 1] * K;
+    else g.value = smax(0.0, v_spline) * K;
+    g.delta = spline_deriv(x, u, cu, n, xs) * (K / S);
+    // gamma: stencil values exist on [1, n-2) only; the rest read as zero
+    int il, ir;
+    if (xs <= x[0]) { il = ir = 0; }
+    else if (xs >= x[n - 1]) { il = ir = n - 1; }
+    else {
+        int lo = 0, hi = n;
+        while (lo < hi) { const int mid = (lo + hi) >> 1; if (x[mid] < xs) lo = mid + 1; else hi = mid; }
+        ir = lo;
+        il = ir > 0 ? ir - 1 : 0;
+        if (fabs(x[ir] - xs) < 1e-14) il = ir;
+    }
+    double d1l = 0.0, d2l = 0.0, d1r = 0.0, d2r = 0.0;
+    if (il >= 1 && il < n - 2) stencil_at(x, u, n, uniform, il, d1l, d2l);
+    if (ir >= 1 && ir < n - 2) stencil_at(x, u, n, uniform, ir, d1r, d2r);
+    double dvs, d2vs;
+    if (il == ir) { dvs = d1l; d2vs = d2l; }
+    else {
+        const double al = (xs - x[il]) / (x[ir] - x[il]);
+        dvs = d1l + al * (d1r - d1l);
+        d2vs = d2l + al * (d2r - d2l);
+    }
+    const double k_over_s2 = K / (S * S);
+    g.gamma = fma(k_over_s2, d2vs, -k_over_s2 * dvs);
+    const double v_prev = spline_eval(x, up, cup, n, xs);
+    g.theta = (v_prev - v_spline) / dt * K;
+    return g;
+}
+
+__device__ bool params_valid(const PdeJob& j) {
+    auto fin = [](double v) { return isfinite(v); };
+    if (!(j.spot > 0.0) || !fin(j.spot)) return false;
+    if (!(j.strike > 0.0) || !fin(j.strike)) return false;
+    if (!(j.maturity > 0.0) || !fin(j.maturity)) return false;
+    if (!fin(j.rate)) return false;
+    if (j.dividend_yield < 0.0 || !fin(j.dividend_yield)) return false;
+    if (!(j.volatility > 0.0) || !fin(j.volatility)) return false;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// The solve kernel
+// ---------------------------------------------------------------------------
+template <int K, int M>
+struct StageCache {
+    double cp[M];
+    double inv[M];
+    uint32_t mask;
+    bool valid;
+    bool singular;
+};
+
+template <int K, int M>
+__global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __restrict__ jobs, int n_jobs,
+                                                                Greeks* __restrict__ greeks_out,
+                                                                int32_t* __restrict__ status_out,
+                                                                StateOut state) {
+    constexpr int NMAX = K * M;
+    constexpr int GPB = kBlock / K;
+    __shared__ double s_x[GPB][NMAX];
+    __shared__ double s_y[GPB][NMAX];
+    __shared__ double s_cp[GPB][NMAX];
+    __shared__ double s_c[GPB][NMAX];
+
+    const int tid = threadIdx.x;
+    const int slot = tid / K;
+    const int lane = tid % K;
+    int job_idx = blockIdx.x * GPB + slot;
+    const bool live = job_idx < n_jobs;
+    if (!live) job_idx = n_jobs - 1;
+    const PdeJob job = jobs[job_idx];
+    const GridDesc& g = job.grid;
+    const int n = static_cast<int>(g.n_points);
+    const bool put = job.option_type == 1;
+    const bool valid = params_valid(job) && n >= 3 && n <= NMAX;
+    const int base = lane * M;
+
+    // ---- grid (GridSpec::generate) ----
+    double* xs = s_x[slot];
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            if (i < n) xs[i] = raw_grid_point(g, i);
+        }
+    }
+    __syncthreads();
+    if (valid && lane == 0 && needs_monotonicity(g)) enforce_monotonicity(xs, n, g.x_min, g.x_max);
+    __syncthreads();
+
+    // ---- per-node constants ----
+    const double r = job.rate;
+    const double q = job.dividend_yield;
+    const double hss = 0.5 * job.volatility * job.volatility;
+    const double drift = r - q - hss;  // BlackScholesPDE::operator() (black_scholes_pde.hpp:72-76)
+    double x_k[M], psi[M], jl[M], jd[M], ju[M];
+    double c_dxl[M], c_dxr[M], c_dxc[M], c_wl[M], c_wr[M];
+    uint32_t lockable = 0;
+    bool nonuni = false;
+    double my_psi_max = -kBig, my_dx_min = kBig;
+    const double expected_dx = valid ? (xs[n - 1] - xs[0]) / static_cast<double>(n - 1) : 1.0;
+    auto payoff = [&](double xv) {
+        return put ? smax(1.0 - exp(xv), 0.0) : smax(exp(xv) - 1.0, 0.0);
+    };
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const int i = base + k;
+        x_k[k] = 0.0; psi[k] = -kBig; jl[k] = jd[k] = ju[k] = 0.0;
+        c_dxl[k] = c_dxr[k] = c_dxc[k] = c_wl[k] = c_wr[k] = 0.0;
+        if (valid && i < n) {
+            x_k[k] = xs[i];
+            psi[k] = payoff(xs[i]);
+            my_psi_max = smax(my_psi_max, psi[k]);
+            if (i >= 1) {
+                const double dx = xs[i] - xs[i - 1];
+                my_dx_min = smin(my_dx_min, dx);
+                if (fabs(dx - expected_dx) > 100.0 * 2.220446049250313e-16) nonuni = true;
+            }
+            if (i >= 1 && i <= n - 2) {
+                const double dl = xs[i] - xs[i - 1];
+                const double dr = xs[i + 1] - xs[i];
+                const double dc = 0.5 * (dl + dr);
+                c_dxl[k] = 1.0 / dl;
+                c_dxr[k] = 1.0 / dr;
+                c_dxc[k] = 1.0 / dc;
+                c_wl[k] = dr / (dl + dr);
+                c_wr[k] = dl / (dl + dr);
+                // SpatialOperator::assemble_jacobian (spatial_operator.hpp:123-164)
+                const double a = hss, b = r - q - hss, c = -r;
+                const double dx_avg = (dl + dr) / 2.0;
+                const double d2_im1 = a / (dl * dx_avg);
+                const double d2_i = -a * (1.0 / dl + 1.0 / dr) / dx_avg;
+                const double d2_ip1 = a / (dr * dx_avg);
+                const double d1_denom = dl + dr;
+                const double d1_im1 = -b * dr / (dl * d1_denom);
+                const double d1_i = b * (dr - dl) / (dl * dr);
+                const double d1_ip1 = b * dl / (dr * d1_denom);
+                jl[k] = d2_im1 + d1_im1;
+                jd[k] = d2_i + d1_i + c;
+                ju[k] = d2_ip1 + d1_ip1;
+            }
+        }
+    }
+    const bool uniform = !group_any<K>(nonuni);
+    const double psi_max = group_max<K>(my_psi_max);
+    const double dx_min = group_min<K>(my_dx_min);
+    double u_dx = 0.0, u_dx_inv = 0.0, u_dx2_inv = 0.0, u_half_inv = 0.0;
+    if (uniform) {
+        u_dx = expected_dx;
+        u_dx_inv = 1.0 / u_dx;
+        u_dx2_inv = u_dx_inv * u_dx_inv;
+        u_half_inv = u_dx_inv * 0.5;
+    }
+
+    // L(v) at interior node k of this lane, given the neighbours vl/vr.
+    auto apply_L = [&](int k, double vl, double v, double vr) {
+        double d2, d1;
+        if (uniform) {
+            d2 = fma(vr + vl, u_dx2_inv, -2.0 * v * u_dx2_inv);
+            d1 = (vr - vl) * u_half_inv;
+        } else {
+            const double fwd = (vr - v) * c_dxr[k];
+            const double bwd = (v - vl) * c_dxl[k];
+            d2 = (fwd - bwd) * c_dxc[k];
+            const double t1 = c_wl[k] * (v - vl) * c_dxl[k];
+            const double t2 = c_wr[k] * (vr - v) * c_dxr[k];
+            d1 = t1 + t2;
+        }
+        return hss * d2 + drift * d1 - r * v;
+    };
+
+    // deep-ITM lock eligibility: psi > 0.95 psi_max and L(psi) < 0 (pde_solver.hpp:631-680)
+    {
+        const double thr = kDeepItmFraction * psi_max;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            if (valid && psi_max > 0.0 && i >= 1 && i <= n - 2 && psi[k] > thr) {
+                const double pl = payoff(xs[i - 1]);
+                const double pr = payoff(xs[i + 1]);
+                if (apply_L(k, pl, psi[k], pr) < 0.0) lockable |= (1u << k);
+            }
+        }
+    }
+
+    // Dirichlet boundary values (american_option.cpp:192-202, 276-288)
+    const double x_first = valid ? xs[0] : 0.0;
+    const double x_last = valid ? xs[n - 1] : 0.0;
+    const double g_left = put ? smax(1.0 - exp(x_first), 0.0) : 0.0;
+    auto g_right = [&](double t) { return put ? 0.0 : exp(x_last) - exp(-r * t); };
+
+    // ---- step count (TimeDomain::from_n_steps; auto: grid_spec_types.cpp:90-99) ----
+    int64_t n_steps = job.n_time;
+    if (n_steps <= 0) {
+        const double dt_est = job.c_t * dx_min;
+        int64_t nt = static_cast<int64_t>(ceil(job.maturity / dt_est));
+        if (nt > job.max_time_steps) nt = job.max_time_steps;
+        if (nt < job.min_time_steps) nt = job.min_time_steps;
+        n_steps = nt;
+    }
+    const double dt = job.maturity / static_cast<double>(n_steps);
+
+    // ---- initial condition (pde_solver.hpp:96-109) ----
+    double u[M], up[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const int i = base + k;
+        double v = psi[k];
+        if (i == 0) v = g_left;
+        if (i == n - 1) v = g_right(0.0);
+        if (i >= n) v = 0.0;
+        u[k] = (i < n) ? smax(v, psi[k]) : 0.0;
+        up[k] = u[k];
+    }
+
+    StageCache<K, M> cache[2];
+    cache[0].valid = cache[1].valid = false;
+    cache[0].mask = cache[1].mask = 0;
+    cache[0].singular = cache[1].singular = false;
+
+    // One projected implicit stage (solve_implicit_stage_projected,
+    // pde_solver.hpp:525-729): bands, Dirichlet rows, deep-ITM locks,
+    // projected Thomas, boundary values. rhs/guess in, u out.
+    auto stage = [&](double t, double w, const double (&rhs)[M], const double (&guess)[M],
+                     StageCache<K, M>& cc) -> bool {
+        double lo[M], dg[M], upb[M], rb[M];
+        uint32_t mask = 0;
+        const double gr = g_right(t);
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            if (i == 0) { lo[k] = 0.0; dg[k] = 1.0; upb[k] = 0.0; rb[k] = g_left; }
+            else if (i == n - 1) { lo[k] = 0.0; dg[k] = 1.0; upb[k] = 0.0; rb[k] = gr; }
+            else if (i >= n) { lo[k] = 0.0; dg[k] = 1.0; upb[k] = 0.0; rb[k] = 0.0; }
+            else {
+                const bool lock = ((lockable >> k) & 1u) && (guess[k] - psi[k] < kExerciseTol);
+                if (lock) {
+                    mask |= (1u << k);
+                    lo[k] = 0.0; dg[k] = 1.0; upb[k] = 0.0; rb[k] = psi[k];
+                } else {
+                    lo[k] = -w * jl[k];
+                    dg[k] = 1.0 - w * jd[k];
+                    upb[k] = -w * ju[k];
+                    rb[k] = rhs[k];
+                }
+            }
+        }
+        // --- c' and 1/denom (cached per stage kind and lock pattern) ---
+        const bool stale = group_any<K>(!cc.valid || mask != cc.mask);
+        if (stale) {
+            double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                // T_k = [[0, up],[-l, d]] applied on the left
+                const double n00 = upb[k] * m10;
+                const double n01 = upb[k] * m11;
+                const double n10 = fma(-lo[k], m00, dg[k] * m10);
+                const double n11 = fma(-lo[k], m01, dg[k] * m11);
+                m00 = n00; m01 = n01; m10 = n10; m11 = n11;
+                renorm(m00, m01, m10, m11);
+            }
+#pragma unroll
+            for (int d = 1; d < K; d <<= 1) {
+                const double p00 = shup<K>(m00, d), p01 = shup<K>(m01, d);
+                const double p10 = shup<K>(m10, d), p11 = shup<K>(m11, d);
+                if (lane >= d) {
+                    const double a00 = fma(m00, p00, m01 * p10);
+                    const double a01 = fma(m00, p01, m01 * p11);
+                    const double a10 = fma(m10, p00, m11 * p10);
+                    const double a11 = fma(m10, p01, m11 * p11);
+                    m00 = a00; m01 = a01; m10 = a10; m11 = a11;
+                    renorm(m00, m01, m10, m11);
+                }
+            }
+            const double e01 = shup<K>(m01, 1);
+            const double e11 = shup<K>(m11, 1);
+            double cprev = (lane == 0) ? 0.0 : e01 / e11;
+            bool sing = false;
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                const double denom = fma(-lo[k], cprev, dg[k]);
+                if (fabs(denom) < kSingularTol) sing = true;
+                const double inv = 1.0 / denom;
+                cc.cp[k] = upb[k] * inv;
+                cc.inv[k] = inv;
+                cprev = cc.cp[k];
+            }
+            cc.singular = group_any<K>(sing);
+            cc.mask = mask;
+            cc.valid = true;
+        }
+        if (cc.singular) return false;
+        // --- d' : affine scan ---
+        double A = 0.0, B = 1.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const double al = rb[k] * cc.inv[k];
+            const double be = -lo[k] * cc.inv[k];
+            A = fma(be, A, al);
+            B = be * B;
+        }
+#pragma unroll
+        for (int d = 1; d < K; d <<= 1) {
+            const double pA = shup<K>(A, d), pB = shup<K>(B, d);
+            if (lane >= d) { A = fma(B, pA, A); B = B * pB; }
+        }
+        double dprev = shup<K>(A, 1);
+        if (lane == 0) dprev = 0.0;
+        double dp[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            dp[k] = fma(-lo[k], dprev, rb[k]) * cc.inv[k];
+            dprev = dp[k];
+        }
+        // --- projected back substitution: clamped-affine suffix scan ---
+        double fa = 1.0, fb = 0.0, flo = -kBig, fhi = kBig;
+#pragma unroll
+        for (int k = M - 1; k >= 0; --k) {
+            const double mm = -cc.cp[k], dd = dp[k], L = psi[k], H = kBig;
+            const double na = mm * fa;
+            const double nb = fma(mm, fb, dd);
+            double nlo, nhi;
+            if (mm >= 0.0) { nlo = clampd(fma(mm, flo, dd), L, H); nhi = clampd(fma(mm, fhi, dd), L, H); }
+            else { nlo = clampd(fma(mm, fhi, dd), L, H); nhi = clampd(fma(mm, flo, dd), L, H); }
+            fa = na; fb = nb; flo = nlo; fhi = nhi;
+        }
+#pragma unroll
+        for (int d = 1; d < K; d <<= 1) {
+            const double qa = shdn<K>(fa, d), qb = shdn<K>(fb, d);
+            const double qlo = shdn<K>(flo, d), qhi = shdn<K>(fhi, d);
+            if (lane + d < K) {
+                // (f o q)(v) with f = (fa, fb, flo, fhi), q = (qa, qb, qlo, qhi)
+                const double na = fa * qa;
+                const double nb = fma(fa, qb, fb);
+                double nlo, nhi;
+                if (fa >= 0.0) { nlo = clampd(fma(fa, qlo, fb), flo, fhi); nhi = clampd(fma(fa, qhi, fb), flo, fhi); }
+                else { nlo = clampd(fma(fa, qhi, fb), flo, fhi); nhi = clampd(fma(fa, qlo, fb), flo, fhi); }
+                fa = na; fb = nb; flo = nlo; fhi = nhi;
+            }
+        }
+        const double sb = shdn<K>(fb, 1), slo = shdn<K>(flo, 1), shi = shdn<K>(fhi, 1);
+        double v = (lane == K - 1) ? 0.0 : clampd(sb, slo, shi);
+        double outv[M];
+#pragma unroll
+        for (int k = M - 1; k >= 0; --k) {
+            const double unconstrained = fma(-cc.cp[k], v, dp[k]);
+            outv[k] = smax(unconstrained, psi[k]);
+            v = outv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            u[k] = outv[k];
+            if (i == 0) u[k] = g_left;
+            if (i == n - 1) u[k] = gr;
+            if (i >= n) u[k] = 0.0;
+        }
+        return true;
+    };
+
+    bool ok = valid;
+    const double gamma = 2.0 - sqrt(2.0);
+    if (!live) n_steps = 0;
+    if (ok && n_steps > 0) {
+        // Rannacher start: two implicit-Euler half steps (pde_solver.hpp:413-447)
+        double rhs[M], guess[M];
+        const double t0 = 0.0;
+        const double t_next = t0 + dt;
+        const double half_dt = 0.5 * dt;
+        const double t_mid = t0 + half_dt;
+#pragma unroll
+        for (int k = 0; k < M; ++k) { up[k] = u[k]; rhs[k] = up[k]; guess[k] = up[k]; }
+        ok = stage(t_mid, half_dt, rhs, guess, cache[0]);
+        if (ok) {
+#pragma unroll
+            for (int k = 0; k < M; ++k) { up[k] = u[k]; rhs[k] = up[k]; guess[k] = up[k]; }
+            ok = stage(t_next, half_dt, rhs, guess, cache[0]);
+        }
+        cache[0].valid = false;
+    }
+    if (ok) {
+        const double w1 = gamma * dt / 2.0;
+        const double omg = 1.0 - gamma;
+        const double tmg = 2.0 - gamma;
+        const double den = gamma * tmg;
+        const double alpha = 1.0 / den;
+        const double beta = -(omg * omg) / den;
+        const double w2 = (1.0 - gamma) * dt / (2.0 - gamma);
+        double t = static_cast<double>(1) * dt;
+        for (int64_t step = 1; step < n_steps; ++step) {
+            const double t_next = static_cast<double>(step + 1) * dt;
+#pragma unroll
+            for (int k = 0; k < M; ++k) up[k] = u[k];
+            // stage 1: rhs = u^n + w1 L(u^n), guess u^n (pde_solver.hpp:342-362)
+            double rhs[M];
+            {
+                const double left = shup<K>(up[M - 1], 1);
+                const double right = shdn<K>(up[0], 1);
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    const int i = base + k;
+                    double lu = 0.0;
+                    if (i >= 1 && i <= n - 2) {
+                        const double vl = (k == 0) ? left : up[k > 0 ? k - 1 : 0];
+                        const double vr = (k == M - 1) ? right : up[k < M - 1 ? k + 1 : M - 1];
+                        lu = apply_L(k, vl, up[k], vr);
+                    }
+                    rhs[k] = fma(w1, lu, up[k]);
+                }
+            }
+            ok = stage(t + gamma * dt, w1, rhs, up, cache[0]);
+            if (!ok) break;
+            // stage 2: rhs = alpha u* + beta u^n, guess u* (pde_solver.hpp:370-395)
+            double ustar[M];
+#pragma unroll
+            for (int k = 0; k < M; ++k) { ustar[k] = u[k]; rhs[k] = fma(alpha, u[k], beta * up[k]); }
+            ok = stage(t_next, w2, rhs, ustar, cache[1]);
+            if (!ok) break;
+            t = t_next;
+        }
+    }
+
+    // ---- epilogue: natural splines and Greeks on one lane ----
+    double* ys = s_y[slot];
+    double* cps = s_cp[slot];
+    double* cs = s_c[slot];
+    const bool write_state = live && (job.flags & kJobWriteState) && state.u != nullptr;
+    const int64_t row = static_cast<int64_t>(job.out_index) * state.stride;
+    if (write_state && ok) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            if (i < n) {
+                state.x[row + i] = xs[i];
+                state.u[row + i] = u[k];
+                state.u_prev[row + i] = up[k];
+            }
+        }
+        if (lane == 0) { state.n_space[job.out_index] = n; state.dt[job.out_index] = dt; }
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) { const int i = base + k; if (i < n && valid) ys[i] = u[k]; }
+    __syncthreads();
+    double v_cur = 0.0;
+    Greeks gk{NAN, NAN, NAN, NAN};
+    bool spl_ok = true;
+    if (lane == 0 && live && ok) {
+        spl_ok = spline_second(xs, ys, n, cps, cs);
+        const double xsp = log(job.spot / job.strike);
+        v_cur = spline_eval(xs, ys, cs, n, xsp);
+        if (write_state) for (int i = 0; i < n; ++i) state.c_u[row + i] = cs[i];
+        if (job.flags & kJobGreeks) {
+            // value, delta, gamma need u (in ys) and its spline (in cs)
+            const bool uni = uniform;
+            Greeks t = greeks_at(xs, ys, cs, ys, cs, n, uni, job.spot, job.strike, dt);
+            gk.value = t.value; gk.delta = t.delta; gk.gamma = t.gamma;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < M; ++k) { const int i = base + k; if (i < n && valid) ys[i] = up[k]; }
+    __syncthreads();
+    if (lane == 0 && live && ok) {
+        spl_ok = spline_second(xs, ys, n, cps, cs) && spl_ok;
+        if (write_state) for (int i = 0; i < n; ++i) state.c_up[row + i] = cs[i];
+        if (job.flags & kJobGreeks) {
+            const double xsp = log(job.spot / job.strike);
+            const double v_prev = spline_eval(xs, ys, cs, n, xsp);
+            gk.theta = (v_prev - v_cur) / dt * job.strike;
+        }
+    }
+    if (lane == 0 && live) {
+        int32_t st = kStatusOk;
+        if (!valid) st = params_valid(job) ? kStatusSolver : kStatusValidation;
+        else if (!ok) st = kStatusSolver;
+        if (st != kStatusOk) gk = Greeks{NAN, NAN, NAN, NAN};
+        if (job.flags & kJobGreeks) greeks_out[job.out_index] = gk;
+        status_out[job.out_index] = st;
+        (void)spl_ok;
+    }
+}
+
+// Greeks of option i of a normalized chain from its group's stored state.
+__global__ __launch_bounds__(kBlock) void chain_epilogue_kernel(const ChainOption* __restrict__ opts, int n,
+                                                                StateOut state,
+                                                                const int32_t* __restrict__ group_status,
+                                                                Greeks* __restrict__ greeks_out,
+                                                                int32_t* __restrict__ status_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const ChainOption o = opts[i];
+    const int32_t gs = group_status[o.group];
+    if (gs != kStatusOk) {
+        greeks_out[o.out_index] = Greeks{NAN, NAN, NAN, NAN};
+        status_out[o.out_index] = gs;
+        return;
+    }
+    const int64_t row = static_cast<int64_t>(o.group) * state.stride;
+    const int ns = static_cast<int>(state.n_space[o.group]);
+    const double* x = state.x + row;
+    const bool uni = grid_uniform(x, ns);
+    greeks_out[o.out_index] = greeks_at(x, state.u + row, state.c_u + row, state.u_prev + row,
+                                        state.c_up + row, ns, uni, o.spot, o.strike, state.dt[o.group]);
+    status_out[o.out_index] = kStatusOk;
+}
+
+__global__ __launch_bounds__(kBlock) void plan_jobs_kernel(const DeviceOptionRecord* __restrict__ opts, int n,
+                                                           DevicePlan plan, PdeJob* __restrict__ jobs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DeviceOptionRecord o = opts[i];
+    PdeJob j;
+    j.maturity = o.maturity;
+    j.volatility = o.volatility;
+    j.rate = o.rate;
+    j.dividend_yield = o.dividend_yield;
+    j.spot = o.spot;
+    j.strike = o.strike;
+    j.c_t = plan.accuracy.c_t;
+    j.pad0 = 0.0;
+    j.option_type = o.option_type;
+    j.max_time_steps = plan.accuracy.max_time_steps;
+    j.min_time_steps = 0;
+    j.out_index = i;
+    j.flags = kJobGreeks;
+    const bool ok = o.spot > 0.0 && o.strike > 0.0 && o.maturity > 0.0 && o.volatility > 0.0 &&
+                    isfinite(o.spot) && isfinite(o.strike) && isfinite(o.maturity) && isfinite(o.volatility);
+    if (plan.mode == 1) {  // EXPLICIT
+        j.grid = plan.spec;
+        j.n_time = plan.n_time;
+    } else {
+        j.grid = ok ? estimate_spatial(o.spot, o.strike, o.maturity, o.volatility, plan.accuracy) : plan.spec;
+        if (plan.mode == 2) {
+            j.n_time = plan.n_time;
+        } else {
+            j.n_time = 0;
+            j.min_time_steps = plan.min_time_steps;
+        }
+    }
+    jobs[i] = j;
+}
+
+template <int K, int M>
+int launch_variant(const PdeJob* d_jobs, int n_jobs, Greeks* d_greeks, int32_t* d_status,
+                   const StateOut& state, hipStream_t stream) {
+    constexpr int GPB = kBlock / K;
+    const int blocks = (n_jobs + GPB - 1) / GPB;
+    hipLaunchKernelGGL((american_solve_kernel<K, M>), dim3(blocks), dim3(kBlock), 0, stream, d_jobs, n_jobs,
+                       d_greeks, d_status, state);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+}  // namespace
+
+bool variant_supported(const SolveVariant& v) {
+    return (v.lanes == 64 && (v.nodes == 2 || v.nodes == 4 || v.nodes == 7)) ||
+           (v.lanes == 32 && v.nodes == 4) || (v.lanes == 16 && v.nodes == 8);
+}
+
+int max_nodes_supported() { return 64 * 7; }
+
+SolveVariant pick_variant(int64_t n_space) {
+    if (n_space <= 128) return {32, 4};
+    if (n_space <= 256) return {64, 4};
+    return {64, 7};
+}
+
+int launch_american_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, Greeks* d_greeks,
+                          int32_t* d_status, const StateOut& state, hipStream_t stream) {
+    if (n_jobs <= 0) return 0;
+    if (v.lanes == 64 && v.nodes == 2) return launch_variant<64, 2>(d_jobs, n_jobs, d_greeks, d_status, state, stream);
+    if (v.lanes == 64 && v.nodes == 4) return launch_variant<64, 4>(d_jobs, n_jobs, d_greeks, d_status, state, stream);
+    if (v.lanes == 64 && v.nodes == 7) return launch_variant<64, 7>(d_jobs, n_jobs, d_greeks, d_status, state, stream);
+    if (v.lanes == 32 && v.nodes == 4) return launch_variant<32, 4>(d_jobs, n_jobs, d_greeks, d_status, state, stream);
+    if (v.lanes == 16 && v.nodes == 8) return launch_variant<16, 8>(d_jobs, n_jobs, d_greeks, d_status, state, stream);
+    return 2;
+}
+
+int launch_plan_jobs(const DeviceOptionRecord* d_opts, int n, const DevicePlan& plan, PdeJob* d_jobs,
+                     hipStream_t stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(plan_jobs_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d_opts, n, plan,
+                       d_jobs);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_chain_epilogue(const ChainOption* d_opts, int n, const StateOut& state,
+                          const int32_t* d_group_status, Greeks* d_greeks, int32_t* d_status,
+                          hipStream_t stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(chain_epilogue_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d_opts, n,
+                       state, d_group_status, d_greeks, d_status);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+}  // namespace mango_amd

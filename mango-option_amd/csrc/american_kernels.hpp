@@ -1,0 +1,108 @@
+// SPDX-License-Identifier: MIT
+//
+// Device-side job and result layouts shared by the kernels and the host planner.
+#pragma once
+
+#include <cstdint>
+
+#include "mango_grid.hpp"
+
+namespace mango_amd {
+
+// One PDE solve. The PDE is strike-free (it runs in x = ln(S/K) on V/K), so a
+// job carries (sigma, r, q, T, type) for the march, a grid, and the
+// (spot, strike) at which the epilogue evaluates value/delta/gamma/theta.
+struct alignas(16) PdeJob {
+    double maturity;
+    double volatility;
+    double rate;
+    double dividend_yield;
+    double spot;
+    double strike;
+    double c_t;               // auto step count: dt = c_t * dx_min (n_time == 0)
+    double pad0;
+    GridDesc grid;
+    int32_t option_type;      // 0 call, 1 put
+    int32_t n_time;           // 0 => derive from the generated grid
+    int64_t max_time_steps;   // cap for the derived step count
+    int64_t min_time_steps;   // floor for the derived step count (0 = none)
+    int32_t out_index;        // slot in the output arrays
+    int32_t flags;            // kJobWriteState | kJobGreeks
+};
+
+constexpr int32_t kJobWriteState = 1;
+constexpr int32_t kJobGreeks = 2;
+
+// Per-job status codes written by the kernels (MangoStatus values).
+constexpr int32_t kStatusOk = 0;
+constexpr int32_t kStatusValidation = 1;
+constexpr int32_t kStatusSolver = 5;
+
+struct Greeks {
+    double value;
+    double delta;
+    double gamma;
+    double theta;
+};
+
+// Full-state output: rows of `stride` doubles per out_index.
+struct StateOut {
+    double* x;       // grid
+    double* u;       // final solution (V/K)
+    double* u_prev;  // previous-step solution
+    double* c_u;     // natural-spline second-derivative/2 coefficients of u
+    double* c_up;    // same for u_prev
+    int64_t* n_space;
+    double* dt;
+    int64_t stride;
+};
+
+// Launch descriptor of the solve kernel family (lanes per option x nodes per lane).
+struct SolveVariant {
+    int lanes;   // K: lanes cooperating on one option (power of two, <= 64)
+    int nodes;   // M: grid nodes held by each lane
+};
+
+// Host launchers (american_kernels.hip).
+int launch_american_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs,
+                          Greeks* d_greeks, int32_t* d_status, const StateOut& state,
+                          hipStream_t stream);
+bool variant_supported(const SolveVariant& v);
+int max_nodes_supported();
+SolveVariant pick_variant(int64_t n_space);
+
+// Per-option epilogue of the normalized chain: Greeks of option i from the
+// stored state of PDE group group_of[i].
+struct ChainOption {
+    double spot;
+    double strike;
+    int32_t group;
+    int32_t out_index;
+};
+int launch_chain_epilogue(const ChainOption* d_opts, int n, const StateOut& state,
+                          const int32_t* d_group_status, Greeks* d_greeks, int32_t* d_status,
+                          hipStream_t stream);
+
+}  // namespace mango_amd
+
+namespace mango_amd {
+
+// Device-side planning of jobs from compact option records.
+struct DevicePlan {
+    int32_t mode;            // MANGO_GRID_MODE_*
+    int32_t n_time;          // EXPLICIT / ACCURACY_FIXED_TIME
+    int64_t min_time_steps;  // ACCURACY
+    Accuracy accuracy;
+    GridDesc spec;           // EXPLICIT
+};
+
+struct DeviceOptionRecord {  // layout of MangoDeviceOption
+    double spot, strike, maturity, volatility, rate, dividend_yield;
+    int32_t option_type;
+    int32_t reserved;
+};
+
+int launch_plan_jobs(const DeviceOptionRecord* d_opts, int n, const DevicePlan& plan, PdeJob* d_jobs,
+                     hipStream_t stream);
+
+}  // namespace mango_amd

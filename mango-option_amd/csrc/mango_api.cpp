@@ -1,0 +1,821 @@
+// SPDX-License-Identifier: MIT
+//
+// Host side of the C ABI (include/mango_amd.h): validation, grid resolution
+// and batch routing exactly as the reference's host code decides them, then
+// job records for the GPU kernels, launches and result copies.
+//
+// Reference behaviour mirrored here:
+//   validate_pricing_params          src/option/option_spec.cpp:117-155
+//   estimate_pde_grid / batch        src/option/grid_spec_types.cpp:37-177
+//   solve_batch routing              src/option/american_option_batch.cpp:272-302
+//   is_normalized_eligible           src/option/american_option_batch.cpp:26-107
+//   group_by_pde_parameters          src/option/american_option_batch.cpp:219-270
+//   solve_regular_batch grid choice  src/option/american_option_batch.cpp:384-440
+//   mango_price_american             src/ffi/mango_c_api.cpp:309-361
+#include "../../include/mango_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "american_kernels.hpp"
+#include "mango_host.hpp"
+
+using namespace mango_amd;
+
+struct MangoAmericanResult {
+    double value, delta, gamma, theta;
+    double spot, strike;
+    std::vector<double> x, u, c_u;
+};
+
+namespace mango_amd {
+
+void set_err(MangoError* err, MangoStatus code, const char* msg) {
+    if (!err) return;
+    err->code = code;
+    if (!msg) { err->message[0] = '\0'; return; }
+    size_t n = std::strlen(msg);
+    if (n > 255) n = 255;
+    std::memcpy(err->message, msg, n);
+    err->message[n] = '\0';
+}
+
+Accuracy to_accuracy(const MangoGridAccuracy* a) {
+    Accuracy acc;
+    acc.alpha = default_sinh_alpha();
+    if (!a) return acc;
+    acc.n_sigma = a->n_sigma;
+    acc.alpha = a->alpha;
+    acc.tol = a->tol;
+    acc.c_t = a->c_t;
+    acc.min_spatial_points = static_cast<int64_t>(a->min_spatial_points);
+    acc.max_spatial_points = static_cast<int64_t>(a->max_spatial_points);
+    acc.max_time_steps = static_cast<int64_t>(a->max_time_steps);
+    return acc;
+}
+
+GridDesc to_desc(const MangoGridSpec& s) {
+    GridDesc g;
+    g.type = s.type;
+    g.x_min = s.x_min;
+    g.x_max = s.x_max;
+    g.n_points = static_cast<int64_t>(s.n_points);
+    g.concentration = s.concentration;
+    g.n_clusters = std::min<int32_t>(s.n_clusters, kMaxClusters);
+    for (int i = 0; i < g.n_clusters; ++i) {
+        g.cl_center[i] = s.clusters[i].center_x;
+        g.cl_alpha[i] = s.clusters[i].alpha;
+        g.cl_weight[i] = s.clusters[i].weight;
+    }
+    return g;
+}
+
+void from_desc(const GridDesc& g, MangoGridSpec* s) {
+    std::memset(s, 0, sizeof(*s));
+    s->type = g.type;
+    s->x_min = g.x_min;
+    s->x_max = g.x_max;
+    s->n_points = static_cast<uint64_t>(g.n_points);
+    s->concentration = g.concentration;
+    s->n_clusters = g.n_clusters;
+    for (int i = 0; i < g.n_clusters; ++i) {
+        s->clusters[i].center_x = g.cl_center[i];
+        s->clusters[i].alpha = g.cl_alpha[i];
+        s->clusters[i].weight = g.cl_weight[i];
+    }
+}
+
+// validate_option_spec + validate_pricing_params; the device path supports a
+// constant rate and no discrete dividends.
+MangoStatus validate_params(const MangoPricingParams& p, const char** why) {
+    auto bad = [&](const char* m) { *why = m; return MANGO_ERR_VALIDATION; };
+    if (p.option_type != MANGO_CALL && p.option_type != MANGO_PUT) return bad("invalid option_type");
+    if (p.spot <= 0.0 || !std::isfinite(p.spot)) return bad("spot price must be positive");
+    if (p.strike <= 0.0 || !std::isfinite(p.strike)) return bad("strike must be positive");
+    if (p.maturity <= 0.0 || !std::isfinite(p.maturity)) return bad("maturity must be positive");
+    if (p.n_tenor_points == 0 && !std::isfinite(p.rate_const)) return bad("invalid rate");
+    if (p.dividend_yield < 0.0 || !std::isfinite(p.dividend_yield)) return bad("invalid dividend yield");
+    if (p.volatility <= 0.0 || !std::isfinite(p.volatility)) return bad("volatility must be positive");
+    if (p.n_tenor_points != 0) { *why = "yield curves are not supported by the GPU solver"; return MANGO_ERR_SOLVER; }
+    if (p.n_dividends != 0) { *why = "discrete dividends are not supported by the GPU solver"; return MANGO_ERR_SOLVER; }
+    *why = nullptr;
+    return MANGO_OK;
+}
+
+// estimate_pde_grid with the step count (needs the generated grid's dx_min).
+std::pair<GridDesc, int64_t> estimate_grid_host(const MangoPricingParams& p, const Accuracy& acc) {
+    GridDesc g = estimate_spatial(p.spot, p.strike, p.maturity, p.volatility, acc);
+    std::vector<double> pts(static_cast<size_t>(g.n_points));
+    generate_grid(g, pts.data());
+    double dx_min = pts[1] - pts[0];
+    for (size_t i = 2; i < pts.size(); ++i) dx_min = std::min(dx_min, pts[i] - pts[i - 1]);
+    const double dt = acc.c_t * dx_min;
+    int64_t nt = static_cast<int64_t>(std::ceil(p.maturity / dt));
+    nt = std::min(nt, acc.max_time_steps);
+    return {g, nt};
+}
+
+std::pair<GridDesc, int64_t> estimate_batch_grid_host(const MangoPricingParams* ps, size_t n, const Accuracy& acc,
+                                                      double* max_maturity) {
+    if (n == 0) {
+        GridDesc g;
+        g.type = kSinh; g.x_min = -3.0; g.x_max = 3.0; g.n_points = 101; g.concentration = acc.alpha;
+        if (max_maturity) *max_maturity = 1.0;
+        return {g, 100};
+    }
+    double gx_min = std::numeric_limits<double>::max();
+    double gx_max = std::numeric_limits<double>::lowest();
+    int64_t gnx = 0, gnt = 0;
+    double max_t = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        auto [g, nt] = estimate_grid_host(ps[i], acc);
+        gx_min = std::min(gx_min, g.x_min);
+        gx_max = std::max(gx_max, g.x_max);
+        gnx = std::max(gnx, g.n_points);
+        gnt = std::max(gnt, nt);
+        max_t = std::max(max_t, ps[i].maturity);
+    }
+    GridDesc g;
+    const double c[1] = {0.0}, a[1] = {acc.alpha}, w[1] = {1.0};
+    if (!make_multi_sinh(g, gx_min, gx_max, gnx, c, a, w, 1)) {
+        g = GridDesc{};
+        g.type = kSinh; g.x_min = gx_min; g.x_max = gx_max; g.n_points = gnx; g.concentration = acc.alpha;
+        g.n_clusters = 0;
+    }
+    if (max_maturity) *max_maturity = max_t;
+    return {g, gnt};
+}
+
+// ---------------------------------------------------------------------------
+// Device context
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_mu;
+int g_device_ok = -1;
+std::string g_device_msg;
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    explicit DevBuf(size_t count) : n(count) {
+        if (count && hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)) != hipSuccess) p = nullptr;
+    }
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    bool ok() const { return n == 0 || p != nullptr; }
+};
+}  // namespace
+
+bool device_ready(MangoError* err) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_device_ok < 0) {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+            g_device_ok = 0;
+            g_device_msg = "no HIP device available: the MI355X solver has no CPU fallback";
+        } else {
+            hipDeviceProp_t prop;
+            int dev = 0;
+            (void)hipGetDevice(&dev);
+            if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+                g_device_ok = 0;
+                g_device_msg = "hipGetDeviceProperties failed";
+            } else if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+                g_device_ok = 0;
+                g_device_msg = std::string("device is ") + prop.gcnArchName + ", kernels are built for gfx950";
+            } else {
+                g_device_ok = 1;
+            }
+        }
+    }
+    if (!g_device_ok) set_err(err, MANGO_ERR_SOLVER, g_device_msg.c_str());
+    return g_device_ok == 1;
+}
+
+// Grow-only device buffer for job records of the device-resident path.
+PdeJob* device_job_scratch(size_t n, MangoError* err) {
+    static PdeJob* buf = nullptr;
+    static size_t cap = 0;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n > cap) {
+        if (buf) (void)hipFree(buf);
+        buf = nullptr;
+        cap = 0;
+        if (hipMalloc(reinterpret_cast<void**>(&buf), n * sizeof(PdeJob)) != hipSuccess) {
+            set_err(err, MANGO_ERR_SOLVER, "hipMalloc failed");
+            return nullptr;
+        }
+        cap = n;
+    }
+    return buf;
+}
+
+// Run jobs on the device, one launch per kernel variant (jobs binned by grid
+// size). greeks/status are indexed by job.out_index (n_out slots).
+MangoStatus run_jobs(std::vector<PdeJob>& jobs, size_t n_out, Greeks* greeks, int32_t* status,
+                     HostState* state, MangoError* err) {
+    if (jobs.empty()) return MANGO_OK;
+    if (!device_ready(err)) return MANGO_ERR_SOLVER;
+    int64_t max_n = 0;
+    for (const auto& j : jobs) max_n = std::max(max_n, j.grid.n_points);
+    if (max_n > max_nodes_supported()) {
+        set_err(err, MANGO_ERR_SOLVER, "grid has more points than the GPU solver supports");
+        return MANGO_ERR_SOLVER;
+    }
+    // bin by variant so each launch is homogeneous
+    std::vector<PdeJob> sorted;
+    sorted.reserve(jobs.size());
+    std::vector<std::pair<SolveVariant, std::pair<size_t, size_t>>> ranges;
+    {
+        std::vector<SolveVariant> var(jobs.size());
+        for (size_t i = 0; i < jobs.size(); ++i) var[i] = pick_variant(std::max<int64_t>(jobs[i].grid.n_points, 3));
+        const SolveVariant all[] = {{32, 4}, {64, 4}, {64, 7}, {64, 2}, {16, 8}};
+        for (const auto& v : all) {
+            const size_t start = sorted.size();
+            for (size_t i = 0; i < jobs.size(); ++i)
+                if (var[i].lanes == v.lanes && var[i].nodes == v.nodes) sorted.push_back(jobs[i]);
+            if (sorted.size() > start) ranges.push_back({v, {start, sorted.size() - start}});
+        }
+    }
+    const int64_t stride = state ? state->stride : 0;
+    DevBuf<PdeJob> d_jobs(sorted.size());
+    DevBuf<Greeks> d_greeks(n_out);
+    DevBuf<int32_t> d_status(n_out);
+    const size_t srows = state ? n_out : 0;
+    DevBuf<double> d_x(srows * stride), d_u(srows * stride), d_up(srows * stride), d_cu(srows * stride),
+        d_cup(srows * stride), d_dt(srows);
+    DevBuf<int64_t> d_ns(srows);
+    if (!d_jobs.ok() || !d_greeks.ok() || !d_status.ok() || !d_x.ok() || !d_u.ok() || !d_up.ok() || !d_cu.ok() ||
+        !d_cup.ok() || !d_dt.ok() || !d_ns.ok()) {
+        set_err(err, MANGO_ERR_SOLVER, "hipMalloc failed");
+        return MANGO_ERR_SOLVER;
+    }
+    hipStream_t stream = nullptr;
+    bool ok = hipMemcpy(d_jobs.p, sorted.data(), sorted.size() * sizeof(PdeJob), hipMemcpyHostToDevice) == hipSuccess;
+    StateOut so{};
+    if (state) {
+        so.x = d_x.p; so.u = d_u.p; so.u_prev = d_up.p; so.c_u = d_cu.p; so.c_up = d_cup.p;
+        so.n_space = d_ns.p; so.dt = d_dt.p; so.stride = stride;
+    }
+    for (const auto& [v, r] : ranges)
+        ok = ok && launch_american_solve(v, d_jobs.p + r.first, static_cast<int>(r.second), d_greeks.p,
+                                         d_status.p, so, stream) == 0;
+    ok = ok && hipStreamSynchronize(stream) == hipSuccess;
+    // Only the slots the jobs wrote are copied back.
+    std::vector<Greeks> hg(n_out);
+    std::vector<int32_t> hs(n_out);
+    ok = ok && hipMemcpy(hg.data(), d_greeks.p, n_out * sizeof(Greeks), hipMemcpyDeviceToHost) == hipSuccess;
+    ok = ok && hipMemcpy(hs.data(), d_status.p, n_out * sizeof(int32_t), hipMemcpyDeviceToHost) == hipSuccess;
+    if (state && ok) {
+        state->x.resize(srows * stride); state->u.resize(srows * stride); state->u_prev.resize(srows * stride);
+        state->c_u.resize(srows * stride); state->c_up.resize(srows * stride);
+        state->n_space.resize(srows); state->dt.resize(srows);
+        ok = ok && hipMemcpy(state->x.data(), d_x.p, srows * stride * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && hipMemcpy(state->u.data(), d_u.p, srows * stride * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && hipMemcpy(state->u_prev.data(), d_up.p, srows * stride * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && hipMemcpy(state->c_u.data(), d_cu.p, srows * stride * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && hipMemcpy(state->c_up.data(), d_cup.p, srows * stride * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && hipMemcpy(state->n_space.data(), d_ns.p, srows * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && hipMemcpy(state->dt.data(), d_dt.p, srows * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (!ok) {
+        set_err(err, MANGO_ERR_SOLVER, (std::string("HIP error: ") + hipGetErrorString(hipGetLastError())).c_str());
+        return MANGO_ERR_SOLVER;
+    }
+    for (const auto& j : sorted) {
+        const size_t o = static_cast<size_t>(j.out_index);
+        if (greeks && (j.flags & kJobGreeks)) greeks[o] = hg[o];
+        if (status) status[o] = hs[o];
+    }
+    return MANGO_OK;
+}
+
+// Normalized chain on the device: group solves with state kept in HBM, then
+// one epilogue thread per option reads its group's splines.
+MangoStatus run_chain(std::vector<PdeJob>& gjobs, const std::vector<ChainOption>& opts, size_t n_out, Greeks* greeks,
+                      int32_t* status, HostState* state, MangoError* err) {
+    if (gjobs.empty()) return MANGO_OK;
+    if (!device_ready(err)) return MANGO_ERR_SOLVER;
+    int64_t stride = 0;
+    for (const auto& j : gjobs) stride = std::max(stride, j.grid.n_points);
+    if (stride > max_nodes_supported()) {
+        set_err(err, MANGO_ERR_SOLVER, "grid has more points than the GPU solver supports");
+        return MANGO_ERR_SOLVER;
+    }
+    const size_t ng = gjobs.size();
+    DevBuf<PdeJob> d_jobs(ng);
+    DevBuf<int32_t> d_gstatus(ng);
+    DevBuf<double> d_x(ng * stride), d_u(ng * stride), d_up(ng * stride), d_cu(ng * stride), d_cup(ng * stride),
+        d_dt(ng);
+    DevBuf<int64_t> d_ns(ng);
+    DevBuf<ChainOption> d_opts(opts.size());
+    DevBuf<Greeks> d_greeks(n_out);
+    DevBuf<int32_t> d_status(n_out);
+    if (!d_jobs.ok() || !d_gstatus.ok() || !d_x.ok() || !d_u.ok() || !d_up.ok() || !d_cu.ok() || !d_cup.ok() ||
+        !d_dt.ok() || !d_ns.ok() || !d_opts.ok() || !d_greeks.ok() || !d_status.ok()) {
+        set_err(err, MANGO_ERR_SOLVER, "hipMalloc failed");
+        return MANGO_ERR_SOLVER;
+    }
+    StateOut so{d_x.p, d_u.p, d_up.p, d_cu.p, d_cup.p, d_ns.p, d_dt.p, stride};
+    hipStream_t stream = nullptr;
+    bool ok = hipMemcpy(d_jobs.p, gjobs.data(), ng * sizeof(PdeJob), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(d_opts.p, opts.data(), opts.size() * sizeof(ChainOption), hipMemcpyHostToDevice) == hipSuccess;
+    // group solves binned by variant (jobs are few; one launch per variant present)
+    std::vector<PdeJob> sorted;
+    std::vector<std::pair<SolveVariant, std::pair<size_t, size_t>>> ranges;
+    {
+        const SolveVariant all[] = {{32, 4}, {64, 4}, {64, 7}};
+        for (const auto& v : all) {
+            const size_t start = sorted.size();
+            for (const auto& j : gjobs) {
+                const SolveVariant pv = pick_variant(j.grid.n_points);
+                if (pv.lanes == v.lanes && pv.nodes == v.nodes) sorted.push_back(j);
+            }
+            if (sorted.size() > start) ranges.push_back({v, {start, sorted.size() - start}});
+        }
+    }
+    ok = ok && hipMemcpy(d_jobs.p, sorted.data(), ng * sizeof(PdeJob), hipMemcpyHostToDevice) == hipSuccess;
+    for (const auto& [v, r] : ranges)
+        ok = ok && launch_american_solve(v, d_jobs.p + r.first, static_cast<int>(r.second), nullptr, d_gstatus.p, so,
+                                         stream) == 0;
+    ok = ok && launch_chain_epilogue(d_opts.p, static_cast<int>(opts.size()), so, d_gstatus.p, d_greeks.p,
+                                     d_status.p, stream) == 0;
+    ok = ok && hipStreamSynchronize(stream) == hipSuccess;
+    std::vector<Greeks> hg(n_out);
+    std::vector<int32_t> hs(n_out);
+    ok = ok && hipMemcpy(hg.data(), d_greeks.p, n_out * sizeof(Greeks), hipMemcpyDeviceToHost) == hipSuccess;
+    ok = ok && hipMemcpy(hs.data(), d_status.p, n_out * sizeof(int32_t), hipMemcpyDeviceToHost) == hipSuccess;
+    if (state && ok) {
+        state->stride = stride;
+        state->x.resize(ng * stride); state->u.resize(ng * stride); state->u_prev.resize(ng * stride);
+        state->c_u.resize(ng * stride); state->c_up.resize(ng * stride); state->n_space.resize(ng); state->dt.resize(ng);
+        ok = ok && hipMemcpy(state->x.data(), d_x.p, ng * stride * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && hipMemcpy(state->u.data(), d_u.p, ng * stride * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && hipMemcpy(state->u_prev.data(), d_up.p, ng * stride * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && hipMemcpy(state->c_u.data(), d_cu.p, ng * stride * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && hipMemcpy(state->c_up.data(), d_cup.p, ng * stride * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && hipMemcpy(state->n_space.data(), d_ns.p, ng * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && hipMemcpy(state->dt.data(), d_dt.p, ng * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (!ok) {
+        set_err(err, MANGO_ERR_SOLVER, (std::string("HIP error: ") + hipGetErrorString(hipGetLastError())).c_str());
+        return MANGO_ERR_SOLVER;
+    }
+    for (const auto& o : opts) {
+        greeks[o.out_index] = hg[o.out_index];
+        status[o.out_index] = hs[o.out_index];
+    }
+    return MANGO_OK;
+}
+
+// AmericanOptionResult::value_at from stored knots and spline coefficients
+// (src/option/american_option_result.cpp:29-49).
+double spline_value_at(const std::vector<double>& x, const std::vector<double>& u, const std::vector<double>& c,
+                       double spot, double strike) {
+    const size_t n = x.size();
+    const double xe = std::log(spot / strike);
+    if (xe <= x[0]) return u[0] * strike;
+    if (xe >= x[n - 1]) return u[n - 1] * strike;
+    size_t i = static_cast<size_t>(std::lower_bound(x.begin(), x.end(), xe) - x.begin());
+    if (i > 0 && (i == n || x[i] > xe)) --i;
+    i = std::min(i, n - 2);
+    const double h = x[i + 1] - x[i];
+    const double a = u[i];
+    const double b = (u[i + 1] - u[i]) / h - h * (2.0 * c[i] + c[i + 1]) / 3.0;
+    const double d = (c[i + 1] - c[i]) / (3.0 * h);
+    const double dx = xe - x[i];
+    const double v = a + b * dx + c[i] * (dx * dx) + d * (dx * dx * dx);
+    return std::max(0.0, v) * strike;
+}
+
+// Device-resident batch: plan kernel + solve kernel(s), all on `stream`.
+MangoStatus device_batch(const MangoDeviceOption* d_opts, uint64_t n, const MangoGridConfig* grid,
+                         MangoAmericanGreeks* d_out, int32_t* d_status, hipStream_t stream, MangoError* err) {
+    if (n == 0) return MANGO_OK;
+    if (!d_opts || !d_out || !d_status || !grid) {
+        set_err(err, MANGO_ERR_VALIDATION, "null argument");
+        return MANGO_ERR_VALIDATION;
+    }
+    if (!device_ready(err)) return MANGO_ERR_SOLVER;
+    DevicePlan plan{};
+    plan.mode = grid->mode;
+    plan.n_time = static_cast<int32_t>(grid->n_time);
+    plan.min_time_steps = static_cast<int64_t>(grid->min_time_steps);
+    plan.accuracy = to_accuracy(&grid->accuracy);
+    plan.spec = to_desc(grid->spec);
+    int64_t n_max;
+    if (grid->mode == MANGO_GRID_MODE_EXPLICIT) {
+        n_max = plan.spec.n_points;
+        if (n_max < 3 || grid->n_time == 0) { set_err(err, MANGO_ERR_VALIDATION, "explicit grid needs n_points >= 3 and n_time > 0"); return MANGO_ERR_VALIDATION; }
+    } else if (grid->mode == MANGO_GRID_MODE_ACCURACY_FIXED_TIME || grid->mode == MANGO_GRID_MODE_ACCURACY) {
+        n_max = plan.accuracy.max_spatial_points;
+        if (n_max % 2 == 0) ++n_max;
+        if (grid->mode == MANGO_GRID_MODE_ACCURACY_FIXED_TIME && grid->n_time == 0) { set_err(err, MANGO_ERR_VALIDATION, "n_time must be > 0"); return MANGO_ERR_VALIDATION; }
+    } else {
+        set_err(err, MANGO_ERR_VALIDATION, "unknown grid mode");
+        return MANGO_ERR_VALIDATION;
+    }
+    if (n_max > max_nodes_supported()) {
+        set_err(err, MANGO_ERR_SOLVER, "grid has more points than the GPU solver supports");
+        return MANGO_ERR_SOLVER;
+    }
+    PdeJob* d_jobs = device_job_scratch(n, err);
+    if (!d_jobs) return MANGO_ERR_SOLVER;
+    if (launch_plan_jobs(reinterpret_cast<const DeviceOptionRecord*>(d_opts), static_cast<int>(n), plan, d_jobs,
+                         stream) != 0 ||
+        launch_american_solve(pick_variant(n_max), d_jobs, static_cast<int>(n), reinterpret_cast<Greeks*>(d_out),
+                              d_status, StateOut{}, stream) != 0) {
+        set_err(err, MANGO_ERR_SOLVER, (std::string("HIP launch error: ") + hipGetErrorString(hipGetLastError())).c_str());
+        return MANGO_ERR_SOLVER;
+    }
+    return MANGO_OK;
+}
+
+PdeJob make_job(const MangoPricingParams& p, const GridDesc& g, int64_t n_time, int32_t out_index, int32_t flags) {
+    PdeJob j{};
+    j.maturity = p.maturity;
+    j.volatility = p.volatility;
+    j.rate = p.rate_const;
+    j.dividend_yield = p.dividend_yield;
+    j.spot = p.spot;
+    j.strike = p.strike;
+    j.c_t = 0.75;
+    j.grid = g;
+    j.option_type = p.option_type == MANGO_PUT ? 1 : 0;
+    j.n_time = static_cast<int32_t>(n_time);
+    j.max_time_steps = 0;
+    j.min_time_steps = 0;
+    j.out_index = out_index;
+    j.flags = flags;
+    return j;
+}
+
+// The batch planner: BatchAmericanOptionSolver::solve_batch routing.
+MangoStatus price_batch(const MangoPricingParams* params, size_t n, bool use_shared_grid, bool use_normalized,
+                        const MangoGridConfig* grid, MangoAmericanGreeks* out, int32_t* out_status,
+                        uint64_t* out_failed, MangoError* err, HostState* full_state) {
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    std::vector<int32_t> status(n, MANGO_ERR_SOLVER);
+    std::vector<Greeks> greeks(n, Greeks{nan, nan, nan, nan});
+    std::vector<char> valid(n, 0);
+    std::string first_msg;
+    for (size_t i = 0; i < n; ++i) {
+        const char* why = nullptr;
+        status[i] = validate_params(params[i], &why);
+        valid[i] = status[i] == MANGO_OK;
+        if (!valid[i] && first_msg.empty() && why) first_msg = why;
+    }
+    Accuracy acc = to_accuracy(nullptr);
+    bool custom = false;
+    GridDesc custom_spec;
+    int64_t custom_nt = 0;
+    int32_t mode = -1;
+    int64_t min_steps = 0;
+    if (grid) {
+        mode = grid->mode;
+        if (mode == MANGO_GRID_MODE_ACCURACY || mode == MANGO_GRID_MODE_ACCURACY_FIXED_TIME)
+            acc = to_accuracy(&grid->accuracy);
+        if (mode == MANGO_GRID_MODE_EXPLICIT) {
+            custom = true;
+            custom_spec = to_desc(grid->spec);
+            custom_nt = static_cast<int64_t>(grid->n_time);
+            if (custom_spec.n_points < 3 || custom_nt <= 0) {
+                set_err(err, MANGO_ERR_VALIDATION, "explicit grid needs n_points >= 3 and n_time > 0");
+                return MANGO_ERR_VALIDATION;
+            }
+        }
+        if (mode == MANGO_GRID_MODE_ACCURACY) min_steps = static_cast<int64_t>(grid->min_time_steps);
+        if (mode != MANGO_GRID_MODE_ACCURACY && mode != MANGO_GRID_MODE_EXPLICIT &&
+            mode != MANGO_GRID_MODE_ACCURACY_FIXED_TIME) {
+            set_err(err, MANGO_ERR_VALIDATION, "unknown grid mode");
+            return MANGO_ERR_VALIDATION;
+        }
+    }
+    if (acc.tol == 0.0) acc = to_accuracy(nullptr);  // solve_batch resets a zero tolerance
+
+    // Compact the valid options (the reference fails invalid ones per slot).
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < n; ++i) if (valid[i]) idx.push_back(i);
+    std::vector<MangoPricingParams> vp;
+    vp.reserve(idx.size());
+    for (size_t i : idx) vp.push_back(params[i]);
+
+    std::vector<PdeJob> jobs;
+    bool normalized = false;
+    if (use_shared_grid && use_normalized && mode != MANGO_GRID_MODE_ACCURACY_FIXED_TIME && !vp.empty() &&
+        idx.size() == n) {
+        // is_normalized_eligible
+        bool elig = true;
+        for (size_t i = 1; i < vp.size(); ++i) if (vp[i].option_type != vp[0].option_type) elig = false;
+        if (elig) {
+            const GridDesc g0 = estimate_spatial(vp[0].spot, vp[0].strike, vp[0].maturity, vp[0].volatility, acc);
+            const double dx = (g0.x_max - g0.x_min) / static_cast<double>(g0.n_points - 1);
+            if (dx > 0.05 || g0.x_max - g0.x_min > 5.8) elig = false;
+            if (elig) {
+                double m_min = std::numeric_limits<double>::infinity(), m_max = -m_min;
+                for (const auto& p : vp) { const double m = p.spot / p.strike; m_min = std::min(m_min, m); m_max = std::max(m_max, m); }
+                const double ml = std::log(m_min) - g0.x_min;
+                const double mr = g0.x_max - std::log(m_max);
+                const double mm = std::max(0.35, 6.0 * dx);
+                if (ml < mm || mr < mm) elig = false;
+            }
+        }
+        normalized = elig;
+    }
+    if (normalized) {
+        // group_by_pde_parameters; one S = K = 1 solve per group on the shared grid
+        struct Group { MangoPricingParams p; std::vector<size_t> members; };
+        std::vector<Group> groups;
+        constexpr double TOL = 1e-10;
+        for (size_t i = 0; i < vp.size(); ++i) {
+            const auto& p = vp[i];
+            bool found = false;
+            for (auto& g : groups) {
+                if (std::abs(g.p.volatility - p.volatility) < TOL && std::abs(g.p.rate_const - p.rate_const) < TOL &&
+                    std::abs(g.p.dividend_yield - p.dividend_yield) < TOL &&
+                    std::abs(g.p.maturity - p.maturity) < TOL && g.p.option_type == p.option_type) {
+                    g.members.push_back(i);
+                    found = true;
+                    break;
+                }
+            }
+            if (!found) {
+                MangoPricingParams np = p;
+                np.spot = 1.0;
+                np.strike = 1.0;
+                groups.push_back({np, {i}});
+            }
+        }
+        std::vector<PdeJob> gjobs;
+        int64_t stride = 0;
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            GridDesc gs;
+            int64_t nt;
+            if (custom) { gs = custom_spec; nt = custom_nt; }
+            else { auto e = estimate_batch_grid_host(&groups[gi].p, 1, acc, nullptr); gs = e.first; nt = e.second; }
+            stride = std::max(stride, gs.n_points);
+            gjobs.push_back(make_job(groups[gi].p, gs, nt, static_cast<int32_t>(gi), kJobWriteState));
+        }
+        std::vector<ChainOption> copts;
+        copts.reserve(vp.size());
+        for (size_t gi = 0; gi < groups.size(); ++gi)
+            for (size_t k : groups[gi].members)
+                copts.push_back({vp[k].spot, vp[k].strike, static_cast<int32_t>(gi), static_cast<int32_t>(idx[k])});
+        HostState st;
+        st.stride = stride;
+        const MangoStatus rs = run_chain(gjobs, copts, n, greeks.data(), status.data(), full_state ? &st : nullptr, err);
+        if (rs != MANGO_OK) return rs;
+        if (full_state) *full_state = std::move(st);
+    } else {
+        // solve_regular_batch: per-option grids or one shared grid
+        GridDesc shared_spec;
+        int64_t shared_nt = 0;
+        if (use_shared_grid) {
+            if (custom) { shared_spec = custom_spec; shared_nt = custom_nt; }
+            else {
+                auto e = estimate_batch_grid_host(vp.data(), vp.size(), acc, nullptr);
+                shared_spec = e.first;
+                shared_nt = e.second;
+            }
+        }
+        int64_t stride = 0;
+        for (size_t k = 0; k < vp.size(); ++k) {
+            const auto& p = vp[k];
+            GridDesc gs;
+            int64_t nt;
+            if (use_shared_grid) { gs = shared_spec; nt = shared_nt; }
+            else if (custom) { gs = custom_spec; nt = custom_nt; }
+            else if (mode == MANGO_GRID_MODE_ACCURACY_FIXED_TIME) {
+                gs = estimate_spatial(p.spot, p.strike, p.maturity, p.volatility, acc);
+                nt = static_cast<int64_t>(grid->n_time);
+            } else {
+                gs = estimate_spatial(p.spot, p.strike, p.maturity, p.volatility, acc);
+                nt = 0;  // derived from the generated grid on the device
+            }
+            PdeJob j = make_job(p, gs, nt, static_cast<int32_t>(idx[k]),
+                                kJobGreeks | (full_state ? kJobWriteState : 0));
+            j.c_t = acc.c_t;
+            j.max_time_steps = acc.max_time_steps;
+            j.min_time_steps = min_steps;
+            stride = std::max(stride, gs.n_points);
+            jobs.push_back(j);
+        }
+        HostState st;
+        st.stride = stride;
+        const MangoStatus rs = run_jobs(jobs, n, greeks.data(), status.data(), full_state ? &st : nullptr, err);
+        if (rs != MANGO_OK) return rs;
+        if (full_state) *full_state = std::move(st);
+    }
+    uint64_t failed = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (status[i] != MANGO_OK) { ++failed; greeks[i] = Greeks{nan, nan, nan, nan}; }
+        if (out) out[i] = {greeks[i].value, greeks[i].delta, greeks[i].gamma, greeks[i].theta};
+        if (out_status) out_status[i] = status[i];
+    }
+    if (out_failed) *out_failed = failed;
+    if (failed && err && !first_msg.empty()) set_err(err, MANGO_OK, first_msg.c_str());
+    return MANGO_OK;
+}
+
+}  // namespace mango_amd
+
+// ---------------------------------------------------------------------------
+// extern "C"
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void mango_grid_config_default(MangoGridConfig* cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->mode = MANGO_GRID_MODE_ACCURACY;
+    cfg->accuracy.n_sigma = 5.0;
+    cfg->accuracy.alpha = default_sinh_alpha();
+    cfg->accuracy.tol = 1e-2;
+    cfg->accuracy.c_t = 0.75;
+    cfg->accuracy.min_spatial_points = 100;
+    cfg->accuracy.max_spatial_points = 1200;
+    cfg->accuracy.max_time_steps = 5000;
+    cfg->spec.type = MANGO_GRID_SINH;
+    cfg->spec.x_min = -3.0;
+    cfg->spec.x_max = 3.0;
+    cfg->spec.n_points = 101;
+    cfg->spec.concentration = default_sinh_alpha();
+    cfg->n_time = 1000;
+}
+
+MangoStatus mango_grid_spec_multi_sinh(double x_min, double x_max, uint64_t n_points,
+                                       const MangoSinhCluster* clusters, uint64_t n_clusters,
+                                       MangoGridSpec* out, MangoError* err) {
+    if (!out || (n_clusters && !clusters) || n_clusters > MANGO_MAX_CLUSTERS) {
+        set_err(err, MANGO_ERR_VALIDATION, "invalid cluster list");
+        return MANGO_ERR_VALIDATION;
+    }
+    double c[kMaxClusters], a[kMaxClusters], w[kMaxClusters];
+    for (uint64_t i = 0; i < n_clusters; ++i) { c[i] = clusters[i].center_x; a[i] = clusters[i].alpha; w[i] = clusters[i].weight; }
+    GridDesc g;
+    if (!make_multi_sinh(g, x_min, x_max, static_cast<int64_t>(n_points), c, a, w, static_cast<int>(n_clusters))) {
+        set_err(err, MANGO_ERR_VALIDATION, "invalid multi-sinh grid specification");
+        return MANGO_ERR_VALIDATION;
+    }
+    from_desc(g, out);
+    return MANGO_OK;
+}
+
+MangoStatus mango_estimate_pde_grid(const MangoPricingParams* params, const MangoGridAccuracy* accuracy,
+                                    MangoGridSpec* out_spec, uint64_t* out_n_time, MangoError* err) {
+    if (!params || !out_spec) { set_err(err, MANGO_ERR_VALIDATION, "null argument"); return MANGO_ERR_VALIDATION; }
+    const char* why = nullptr;
+    const MangoStatus s = validate_params(*params, &why);
+    if (s == MANGO_ERR_VALIDATION) { set_err(err, s, why); return s; }
+    auto [g, nt] = estimate_grid_host(*params, to_accuracy(accuracy));
+    from_desc(g, out_spec);
+    if (out_n_time) *out_n_time = static_cast<uint64_t>(nt);
+    return MANGO_OK;
+}
+
+MangoStatus mango_estimate_batch_pde_grid(const MangoPricingParams* params, uint64_t n,
+                                          const MangoGridAccuracy* accuracy, MangoGridSpec* out_spec,
+                                          uint64_t* out_n_time, MangoError* err) {
+    if ((n && !params) || !out_spec) { set_err(err, MANGO_ERR_VALIDATION, "null argument"); return MANGO_ERR_VALIDATION; }
+    auto [g, nt] = estimate_batch_grid_host(params, n, to_accuracy(accuracy), nullptr);
+    from_desc(g, out_spec);
+    if (out_n_time) *out_n_time = static_cast<uint64_t>(nt);
+    return MANGO_OK;
+}
+
+MangoStatus mango_grid_generate(const MangoGridSpec* spec, double* out_x, MangoError* err) {
+    if (!spec || !out_x || spec->n_points < 2) { set_err(err, MANGO_ERR_VALIDATION, "invalid grid"); return MANGO_ERR_VALIDATION; }
+    generate_grid(to_desc(*spec), out_x);
+    return MANGO_OK;
+}
+
+MangoStatus mango_price_american_batch_ex(const MangoPricingParams* params, uint64_t n, int32_t use_shared_grid,
+                                          int32_t use_normalized, const MangoGridConfig* grid,
+                                          MangoAmericanGreeks* out, int32_t* out_status, uint64_t* out_failed,
+                                          MangoError* err) {
+    if (n && (!params || !out || !out_status)) {
+        set_err(err, MANGO_ERR_VALIDATION, "null params or output");
+        return MANGO_ERR_VALIDATION;
+    }
+    if (out_failed) *out_failed = 0;
+    if (n == 0) return MANGO_OK;
+    try {
+        return price_batch(params, n, use_shared_grid != 0, use_normalized != 0, grid, out, out_status, out_failed,
+                           err, nullptr);
+    } catch (const std::exception& ex) {
+        set_err(err, MANGO_ERR_SOLVER, ex.what());
+        return MANGO_ERR_SOLVER;
+    } catch (...) {
+        set_err(err, MANGO_ERR_SOLVER, "unknown error");
+        return MANGO_ERR_SOLVER;
+    }
+}
+
+MangoStatus mango_price_american_batch(const MangoPricingParams* params, uint64_t n, int32_t use_shared_grid,
+                                       const MangoGridConfig* grid, MangoAmericanGreeks* out, int32_t* out_status,
+                                       uint64_t* out_failed, MangoError* err) {
+    return mango_price_american_batch_ex(params, n, use_shared_grid, 1, grid, out, out_status, out_failed, err);
+}
+
+MangoStatus mango_price_american_batch_state(const MangoPricingParams* params, uint64_t n,
+                                             const MangoGridConfig* grid, uint64_t stride, double* out_x,
+                                             double* out_u, double* out_u_prev, uint64_t* out_n_space,
+                                             double* out_dt, MangoAmericanGreeks* out, int32_t* out_status,
+                                             uint64_t* out_failed, MangoError* err) {
+    if (n && (!params || !out || !out_status || !out_x || !out_u || !out_u_prev || !out_n_space || !out_dt)) {
+        set_err(err, MANGO_ERR_VALIDATION, "null params or output");
+        return MANGO_ERR_VALIDATION;
+    }
+    if (out_failed) *out_failed = 0;
+    if (n == 0) return MANGO_OK;
+    try {
+        HostState st;
+        const MangoStatus s = price_batch(params, n, false, false, grid, out, out_status, out_failed, err, &st);
+        if (s != MANGO_OK) return s;
+        for (uint64_t i = 0; i < n; ++i) {
+            out_n_space[i] = 0;
+            out_dt[i] = std::numeric_limits<double>::quiet_NaN();
+            if (out_status[i] != MANGO_OK) continue;
+            const uint64_t ns = static_cast<uint64_t>(st.n_space[i]);
+            if (ns > stride) { set_err(err, MANGO_ERR_VALIDATION, "stride smaller than a grid"); return MANGO_ERR_VALIDATION; }
+            out_n_space[i] = ns;
+            out_dt[i] = st.dt[i];
+            const size_t r = i * static_cast<size_t>(st.stride);
+            std::copy(st.x.begin() + r, st.x.begin() + r + ns, out_x + i * stride);
+            std::copy(st.u.begin() + r, st.u.begin() + r + ns, out_u + i * stride);
+            std::copy(st.u_prev.begin() + r, st.u_prev.begin() + r + ns, out_u_prev + i * stride);
+        }
+        return MANGO_OK;
+    } catch (const std::exception& ex) {
+        set_err(err, MANGO_ERR_SOLVER, ex.what());
+        return MANGO_ERR_SOLVER;
+    }
+}
+
+MangoStatus mango_price_american(const MangoPricingParams* params, MangoAmericanResult** out_result,
+                                 MangoError* out_err) {
+    if (!params || !out_result) {
+        set_err(out_err, MANGO_ERR_VALIDATION, "null params or out_result");
+        return MANGO_ERR_VALIDATION;
+    }
+    *out_result = nullptr;
+    try {
+        const char* why = nullptr;
+        const MangoStatus vs = validate_params(*params, &why);
+        if (vs != MANGO_OK) { set_err(out_err, vs, why); return vs; }
+        MangoAmericanGreeks g;
+        int32_t st = MANGO_ERR_SOLVER;
+        uint64_t failed = 0;
+        HostState state;
+        const MangoStatus s = price_batch(params, 1, false, false, nullptr, &g, &st, &failed, out_err, &state);
+        if (s != MANGO_OK) return s;
+        if (st != MANGO_OK) { set_err(out_err, st, "American option solve failed"); return st; }
+        auto* r = new MangoAmericanResult{};
+        r->value = g.value; r->delta = g.delta; r->gamma = g.gamma; r->theta = g.theta;
+        r->spot = params->spot; r->strike = params->strike;
+        const size_t ns = static_cast<size_t>(state.n_space[0]);
+        r->x.assign(state.x.begin(), state.x.begin() + ns);
+        r->u.assign(state.u.begin(), state.u.begin() + ns);
+        r->c_u.assign(state.c_u.begin(), state.c_u.begin() + ns);
+        *out_result = r;
+        return MANGO_OK;
+    } catch (const std::exception& ex) {
+        set_err(out_err, MANGO_ERR_SOLVER, ex.what());
+        return MANGO_ERR_SOLVER;
+    } catch (...) {
+        set_err(out_err, MANGO_ERR_SOLVER, "unknown error");
+        return MANGO_ERR_SOLVER;
+    }
+}
+
+double mango_american_value(const MangoAmericanResult* r) { return r ? r->value : std::nan(""); }
+double mango_american_delta(const MangoAmericanResult* r) { return r ? r->delta : std::nan(""); }
+double mango_american_gamma(const MangoAmericanResult* r) { return r ? r->gamma : std::nan(""); }
+double mango_american_theta(const MangoAmericanResult* r) { return r ? r->theta : std::nan(""); }
+
+MangoStatus mango_american_value_at(const MangoAmericanResult* r, double spot, double* out, MangoError* out_err) {
+    if (!r || !out) { set_err(out_err, MANGO_ERR_VALIDATION, "null result or out"); return MANGO_ERR_VALIDATION; }
+    *out = spline_value_at(r->x, r->u, r->c_u, spot, r->strike);
+    return MANGO_OK;
+}
+
+void mango_american_result_free(MangoAmericanResult* r) { delete r; }
+
+MangoStatus mango_price_american_batch_device(const MangoDeviceOption* d_opts, uint64_t n, const MangoGridConfig* grid,
+                                              MangoAmericanGreeks* d_out, int32_t* d_status, void* stream,
+                                              MangoError* err) {
+    return device_batch(d_opts, n, grid, d_out, d_status, static_cast<hipStream_t>(stream), err);
+}
+
+}  // extern "C"

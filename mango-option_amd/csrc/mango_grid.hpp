@@ -1,0 +1,223 @@
+// SPDX-License-Identifier: MIT
+//
+// Grid specification math shared by the host planner and the device kernels
+// (__host__ __device__). Restates, in this library's own code:
+//   GridSpec::multi_sinh_spaced validation + merge_nearby_clusters
+//       (src/pde/core/grid.hpp:111-165, 191-235)
+//   GridSpec::generate point formulas (src/pde/core/grid.hpp:364-498)
+//   estimate_pde_grid spatial part (src/option/grid_spec_types.cpp:37-83)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace mango_amd {
+
+constexpr int kMaxClusters = 4;
+
+enum GridType : int32_t { kUniform = 0, kLogSpaced = 1, kSinh = 2, kMultiSinh = 3 };
+
+// Plain description of a GridSpec (clusters already merged).
+struct GridDesc {
+    int32_t type = kSinh;
+    int32_t n_clusters = 0;
+    int64_t n_points = 101;
+    double x_min = -3.0;
+    double x_max = 3.0;
+    double concentration = 1.0;
+    double cl_center[kMaxClusters] = {0, 0, 0, 0};
+    double cl_alpha[kMaxClusters] = {0, 0, 0, 0};
+    double cl_weight[kMaxClusters] = {0, 0, 0, 0};
+};
+
+// GridAccuracyParams defaults (src/option/grid_spec_types.hpp:23-53).
+struct Accuracy {
+    double n_sigma = 5.0;
+    double alpha = 0.0;  // 0 => 2*asinh(n_sigma/sqrt(2)) filled by the caller
+    double tol = 1e-2;
+    double c_t = 0.75;
+    int64_t min_spatial_points = 100;
+    int64_t max_spatial_points = 1200;
+    int64_t max_time_steps = 5000;
+};
+
+__host__ __device__ inline double default_sinh_alpha() { return 2.0 * asinh(5.0 / sqrt(2.0)); }
+
+// merge_nearby_clusters: repeatedly merge the first pair (i<j) within
+// 0.3/mean(alpha) (+100 eps) into a weight-averaged cluster.
+__host__ __device__ inline void merge_clusters(GridDesc& g) {
+    if (g.n_clusters <= 1) return;
+    bool merged = true;
+    while (merged) {
+        merged = false;
+        for (int i = 0; i < g.n_clusters && !merged; ++i) {
+            for (int j = i + 1; j < g.n_clusters && !merged; ++j) {
+                const double delta_x = fabs(g.cl_center[i] - g.cl_center[j]);
+                const double alpha_avg = (g.cl_alpha[i] + g.cl_alpha[j]) / 2.0;
+                const double threshold = 0.3 / alpha_avg;
+                const double epsilon = 100.0 * 2.220446049250313e-16;
+                if (delta_x <= threshold + epsilon) {
+                    const double tw = g.cl_weight[i] + g.cl_weight[j];
+                    const double wi = g.cl_weight[i] / tw;
+                    const double wj = g.cl_weight[j] / tw;
+                    const double c = wi * g.cl_center[i] + wj * g.cl_center[j];
+                    const double a = wi * g.cl_alpha[i] + wj * g.cl_alpha[j];
+                    g.cl_center[i] = c;
+                    g.cl_alpha[i] = a;
+                    g.cl_weight[i] = tw;
+                    for (int k = j; k + 1 < g.n_clusters; ++k) {
+                        g.cl_center[k] = g.cl_center[k + 1];
+                        g.cl_alpha[k] = g.cl_alpha[k + 1];
+                        g.cl_weight[k] = g.cl_weight[k + 1];
+                    }
+                    --g.n_clusters;
+                    merged = true;
+                    break;
+                }
+            }
+        }
+    }
+}
+
+// Validation of GridSpec::multi_sinh_spaced; returns false when invalid.
+__host__ __device__ inline bool make_multi_sinh(GridDesc& g, double x_min, double x_max, int64_t n,
+                                                const double* center, const double* alpha,
+                                                const double* weight, int nc) {
+    if (n < 2 || !(x_min < x_max) || nc <= 0 || nc > kMaxClusters) return false;
+    for (int i = 0; i < nc; ++i) {
+        if (alpha[i] <= 0 || weight[i] <= 0) return false;
+        if (center[i] < x_min || center[i] > x_max) return false;
+    }
+    g.type = kMultiSinh;
+    g.x_min = x_min;
+    g.x_max = x_max;
+    g.n_points = n;
+    g.concentration = 1.0;
+    g.n_clusters = nc;
+    for (int i = 0; i < nc; ++i) {
+        g.cl_center[i] = center[i];
+        g.cl_alpha[i] = alpha[i];
+        g.cl_weight[i] = weight[i];
+    }
+    merge_clusters(g);
+    return true;
+}
+
+// Spatial part of estimate_pde_grid (no discrete dividends).
+__host__ __device__ inline GridDesc estimate_spatial(double spot, double strike, double maturity,
+                                                     double vol, const Accuracy& acc) {
+    const double sigma_sqrt_T = vol * sqrt(maturity);
+    const double x0 = log(spot / strike);
+    const double x_min = x0 - acc.n_sigma * sigma_sqrt_T;
+    const double x_max = x0 + acc.n_sigma * sigma_sqrt_T;
+    const double dx_target = vol * sqrt(acc.tol);
+    int64_t nx = static_cast<int64_t>(ceil((x_max - x_min) / dx_target));
+    if (nx < acc.min_spatial_points) nx = acc.min_spatial_points;
+    if (nx > acc.max_spatial_points) nx = acc.max_spatial_points;
+    if (nx % 2 == 0) nx++;
+    double c[2], a[2], w[2];
+    int nc = 0;
+    c[nc] = x0; a[nc] = acc.alpha; w[nc] = 0.5; ++nc;
+    if (0.0 >= x_min && 0.0 <= x_max) { c[nc] = 0.0; a[nc] = acc.alpha; w[nc] = 1.0; ++nc; }
+    GridDesc g;
+    if (!make_multi_sinh(g, x_min, x_max, nx, c, a, w, nc)) {
+        g = GridDesc{};
+        g.type = kSinh;
+        g.x_min = x_min;
+        g.x_max = x_max;
+        g.n_points = nx;
+        g.concentration = acc.alpha;
+        g.n_clusters = 0;
+    }
+    return g;
+}
+
+// Whether generate() runs enforce_monotonicity for this spec.
+__host__ __device__ inline bool needs_monotonicity(const GridDesc& g) {
+    if (g.type != kMultiSinh) return false;
+    if (g.n_clusters != 1) return true;
+    const double eta_center = (g.cl_center[0] - g.x_min) / (g.x_max - g.x_min);
+    return !(fabs(eta_center - 0.5) < 1e-10);
+}
+
+// Point i of GridSpec::generate before enforce_monotonicity.
+__host__ __device__ inline double raw_grid_point(const GridDesc& g, int64_t i) {
+    const int64_t n = g.n_points;
+    switch (g.type) {
+        case kUniform: {
+            const double dx = (g.x_max - g.x_min) / static_cast<double>(n - 1);
+            return g.x_min + static_cast<double>(i) * dx;
+        }
+        case kLogSpaced: {
+            const double log_min = log(g.x_min);
+            const double log_max = log(g.x_max);
+            const double d_log = (log_max - log_min) / static_cast<double>(n - 1);
+            return exp(log_min + static_cast<double>(i) * d_log);
+        }
+        case kSinh: {
+            const double c = g.concentration;
+            const double sinh_half_c = sinh(c / 2.0);
+            const double eta = static_cast<double>(i) / static_cast<double>(n - 1);
+            const double sinh_term = sinh(c * (eta - 0.5)) / sinh_half_c;
+            const double normalized = (1.0 + sinh_term) / 2.0;
+            return g.x_min + (g.x_max - g.x_min) * normalized;
+        }
+        default: {
+            const double range = g.x_max - g.x_min;
+            const double eta = static_cast<double>(i) / static_cast<double>(n - 1);
+            if (g.n_clusters == 1) {
+                const double c = g.cl_alpha[0];
+                const double center = g.cl_center[0];
+                const double sinh_half_c = sinh(c / 2.0);
+                const double eta_center = (center - g.x_min) / range;
+                if (fabs(eta_center - 0.5) < 1e-10) {
+                    const double sinh_term = sinh(c * (eta - 0.5)) / sinh_half_c;
+                    const double normalized = (1.0 + sinh_term) / 2.0;
+                    return g.x_min + range * normalized;
+                }
+                const double offset = center - (g.x_min + g.x_max) / 2.0;
+                const double sinh_term = sinh(c * (eta - eta_center)) / sinh_half_c;
+                const double normalized = (1.0 + sinh_term) / 2.0;
+                return g.x_min + range * normalized + offset;
+            }
+            double total_weight = 0.0;
+            for (int k = 0; k < g.n_clusters; ++k) total_weight += g.cl_weight[k];
+            double weighted_x = 0.0;
+            for (int k = 0; k < g.n_clusters; ++k) {
+                const double c = g.cl_alpha[k];
+                const double center = g.cl_center[k];
+                const double w = g.cl_weight[k] / total_weight;
+                const double sinh_half_c = sinh(c / 2.0);
+                const double eta_center = (center - g.x_min) / range;
+                const double sinh_term = sinh(c * (eta - eta_center)) / sinh_half_c;
+                const double normalized = (1.0 + sinh_term) / 2.0;
+                const double offset = center - (g.x_min + g.x_max) / 2.0;
+                const double x_i = g.x_min + range * normalized + offset;
+                weighted_x += w * x_i;
+            }
+            return weighted_x;
+        }
+    }
+}
+
+// enforce_monotonicity (src/pde/core/grid.hpp:237-262), serial.
+__host__ __device__ inline void enforce_monotonicity(double* p, int64_t n, double x_min, double x_max) {
+    if (n < 2) return;
+    const double min_spacing = (x_max - x_min) / static_cast<double>(n * 100);
+    p[0] = x_min;
+    p[n - 1] = x_max;
+    for (int64_t i = 1; i < n - 1; ++i)
+        if (p[i] <= p[i - 1] + min_spacing) p[i] = p[i - 1] + min_spacing;
+    for (int64_t i = n - 2; i > 0; --i)
+        if (p[i] >= p[i + 1] - min_spacing) p[i] = p[i + 1] - min_spacing;
+}
+
+// Full host-side generate().
+inline void generate_grid(const GridDesc& g, double* out) {
+    for (int64_t i = 0; i < g.n_points; ++i) out[i] = raw_grid_point(g, i);
+    if (needs_monotonicity(g)) enforce_monotonicity(out, g.n_points, g.x_min, g.x_max);
+}
+
+}  // namespace mango_amd

@@ -1,0 +1,189 @@
+# SPDX-License-Identifier: MIT
+"""ctypes mirror of include/mango_amd.h and the loader of libmango_amd.so.
+
+The struct layouts are the reference's C ABI layouts (src/ffi/mango_c_api.h:
+28-44, asserted at :195-214) plus this library's batch additions. The
+library is loaded from this package directory; there is no fallback: if it is
+missing the import of the compute entry points fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+MANGO_OK = 0
+MANGO_ERR_VALIDATION = 1
+MANGO_ERR_ARBITRAGE = 2
+MANGO_ERR_NO_CONVERGENCE = 3
+MANGO_ERR_BRACKETING = 4
+MANGO_ERR_SOLVER = 5
+
+MANGO_CALL = 0
+MANGO_PUT = 1
+
+MANGO_GRID_UNIFORM = 0
+MANGO_GRID_LOG_SPACED = 1
+MANGO_GRID_SINH = 2
+MANGO_GRID_MULTI_SINH = 3
+MANGO_MAX_CLUSTERS = 4
+
+MANGO_GRID_MODE_ACCURACY = 0
+MANGO_GRID_MODE_EXPLICIT = 1
+MANGO_GRID_MODE_ACCURACY_FIXED_TIME = 2
+
+
+class MangoError(C.Structure):
+    _fields_ = [("code", C.c_int32), ("message", C.c_char * 256)]
+
+
+class MangoDividend(C.Structure):
+    _fields_ = [("calendar_time", C.c_double), ("amount", C.c_double)]
+
+
+class MangoTenorPoint(C.Structure):
+    _fields_ = [("tenor", C.c_double), ("log_discount", C.c_double)]
+
+
+class MangoPricingParams(C.Structure):
+    _fields_ = [
+        ("spot", C.c_double),
+        ("strike", C.c_double),
+        ("maturity", C.c_double),
+        ("dividend_yield", C.c_double),
+        ("volatility", C.c_double),
+        ("rate_const", C.c_double),
+        ("tenor_points", C.POINTER(MangoTenorPoint)),
+        ("n_tenor_points", C.c_uint64),
+        ("dividends", C.POINTER(MangoDividend)),
+        ("n_dividends", C.c_uint64),
+        ("option_type", C.c_int32),
+    ]
+
+
+class MangoGridAccuracy(C.Structure):
+    _fields_ = [
+        ("n_sigma", C.c_double),
+        ("alpha", C.c_double),
+        ("tol", C.c_double),
+        ("c_t", C.c_double),
+        ("min_spatial_points", C.c_uint64),
+        ("max_spatial_points", C.c_uint64),
+        ("max_time_steps", C.c_uint64),
+    ]
+
+
+class MangoSinhCluster(C.Structure):
+    _fields_ = [("center_x", C.c_double), ("alpha", C.c_double), ("weight", C.c_double)]
+
+
+class MangoGridSpec(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("n_clusters", C.c_int32),
+        ("x_min", C.c_double),
+        ("x_max", C.c_double),
+        ("n_points", C.c_uint64),
+        ("concentration", C.c_double),
+        ("clusters", MangoSinhCluster * MANGO_MAX_CLUSTERS),
+    ]
+
+
+class MangoGridConfig(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32),
+        ("reserved", C.c_int32),
+        ("accuracy", MangoGridAccuracy),
+        ("spec", MangoGridSpec),
+        ("n_time", C.c_uint64),
+        ("min_time_steps", C.c_uint64),
+    ]
+
+
+class MangoAmericanGreeks(C.Structure):
+    _fields_ = [("value", C.c_double), ("delta", C.c_double), ("gamma", C.c_double), ("theta", C.c_double)]
+
+
+class MangoDeviceOption(C.Structure):
+    _fields_ = [
+        ("spot", C.c_double),
+        ("strike", C.c_double),
+        ("maturity", C.c_double),
+        ("volatility", C.c_double),
+        ("rate", C.c_double),
+        ("dividend_yield", C.c_double),
+        ("option_type", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+assert C.sizeof(MangoPricingParams) == 88
+assert MangoPricingParams.rate_const.offset == 40
+assert MangoPricingParams.option_type.offset == 80
+assert C.sizeof(MangoError) == 260
+assert C.sizeof(MangoGridAccuracy) == 56
+assert C.sizeof(MangoGridSpec) == 136
+assert C.sizeof(MangoGridConfig) == 216
+assert C.sizeof(MangoAmericanGreeks) == 32
+assert C.sizeof(MangoDeviceOption) == 56
+
+LIB_NAME = "libmango_amd.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+# name -> (restype, argtypes); every symbol include/mango_amd.h declares
+_P = C.POINTER
+SIGNATURES = {
+    "mango_price_american": (C.c_int32, [_P(MangoPricingParams), _P(C.c_void_p), _P(MangoError)]),
+    "mango_american_value": (C.c_double, [C.c_void_p]),
+    "mango_american_delta": (C.c_double, [C.c_void_p]),
+    "mango_american_gamma": (C.c_double, [C.c_void_p]),
+    "mango_american_theta": (C.c_double, [C.c_void_p]),
+    "mango_american_value_at": (C.c_int32, [C.c_void_p, C.c_double, _P(C.c_double), _P(MangoError)]),
+    "mango_american_result_free": (None, [C.c_void_p]),
+    "mango_grid_config_default": (None, [_P(MangoGridConfig)]),
+    "mango_grid_spec_multi_sinh": (C.c_int32, [C.c_double, C.c_double, C.c_uint64, _P(MangoSinhCluster),
+                                               C.c_uint64, _P(MangoGridSpec), _P(MangoError)]),
+    "mango_estimate_pde_grid": (C.c_int32, [_P(MangoPricingParams), _P(MangoGridAccuracy), _P(MangoGridSpec),
+                                            _P(C.c_uint64), _P(MangoError)]),
+    "mango_estimate_batch_pde_grid": (C.c_int32, [_P(MangoPricingParams), C.c_uint64, _P(MangoGridAccuracy),
+                                                  _P(MangoGridSpec), _P(C.c_uint64), _P(MangoError)]),
+    "mango_grid_generate": (C.c_int32, [_P(MangoGridSpec), _P(C.c_double), _P(MangoError)]),
+    "mango_price_american_batch": (C.c_int32, [_P(MangoPricingParams), C.c_uint64, C.c_int32, _P(MangoGridConfig),
+                                               _P(MangoAmericanGreeks), _P(C.c_int32), _P(C.c_uint64),
+                                               _P(MangoError)]),
+    "mango_price_american_batch_ex": (C.c_int32, [_P(MangoPricingParams), C.c_uint64, C.c_int32, C.c_int32,
+                                                  _P(MangoGridConfig), _P(MangoAmericanGreeks), _P(C.c_int32),
+                                                  _P(C.c_uint64), _P(MangoError)]),
+    "mango_price_american_batch_state": (C.c_int32, [_P(MangoPricingParams), C.c_uint64, _P(MangoGridConfig),
+                                                     C.c_uint64, _P(C.c_double), _P(C.c_double), _P(C.c_double),
+                                                     _P(C.c_uint64), _P(C.c_double), _P(MangoAmericanGreeks),
+                                                     _P(C.c_int32), _P(C.c_uint64), _P(MangoError)]),
+    "mango_price_american_batch_device": (C.c_int32, [C.c_void_p, C.c_uint64, _P(MangoGridConfig), C.c_void_p,
+                                                      C.c_void_p, C.c_void_p, _P(MangoError)]),
+}
+
+_lib = None
+
+
+def load_library(path: str | None = None) -> C.CDLL:
+    """Load libmango_amd.so (built in-tree by `make -C mango-option_amd`)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"{LIB_NAME} is not built ({p}); run `make -C mango-option_amd` "
+            "(the GPU solver has no CPU fallback)")
+    lib = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def default_sinh_alpha(n_sigma: float = 5.0) -> float:
+    import math
+    return 2.0 * math.asinh(n_sigma / math.sqrt(2.0))

@@ -1,0 +1,141 @@
+# SPDX-License-Identifier: MIT
+"""Seeded synthetic workloads of the benchmark configurations.
+
+PRNG: xoshiro256** seeded through splitmix64, uniform draws
+u = (next() >> 11) * 2**-53, draws taken in a fixed order per option. The
+same generator is used by the tests, the benchmark and the CPU baseline so
+that every leg sees identical inputs.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+_M64 = (1 << 64) - 1
+
+
+def _rotl(x: int, k: int) -> int:
+    return ((x << k) | (x >> (64 - k))) & _M64
+
+
+class Xoshiro256ss:
+    def __init__(self, seed: int):
+        s = seed & _M64
+        st = []
+        for _ in range(4):  # splitmix64
+            s = (s + 0x9E3779B97F4A7C15) & _M64
+            z = s
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+            st.append(z ^ (z >> 31))
+        self.s = st
+
+    def next(self) -> int:
+        s = self.s
+        result = (_rotl((s[1] * 5) & _M64, 7) * 9) & _M64
+        t = (s[1] << 17) & _M64
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = _rotl(s[3], 45)
+        return result
+
+    def uniform(self) -> float:
+        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+
+
+PUT = 1
+CALL = 0
+
+# numpy structured layout of MangoDeviceOption (56 bytes)
+DEVICE_OPTION_DTYPE = np.dtype([("spot", "<f8"), ("strike", "<f8"), ("maturity", "<f8"), ("volatility", "<f8"),
+                                ("rate", "<f8"), ("dividend_yield", "<f8"), ("option_type", "<i4"),
+                                ("reserved", "<i4")])
+
+
+@dataclass
+class OptionBatch:
+    """Struct-of-arrays option batch (constant rate, continuous yield)."""
+    spot: np.ndarray
+    strike: np.ndarray
+    maturity: np.ndarray
+    volatility: np.ndarray
+    rate: np.ndarray
+    dividend_yield: np.ndarray
+    option_type: np.ndarray  # int32, 0 call / 1 put
+
+    def __len__(self) -> int:
+        return len(self.spot)
+
+    def records(self) -> np.ndarray:
+        rec = np.zeros(len(self), dtype=DEVICE_OPTION_DTYPE)
+        for f in ("spot", "strike", "maturity", "volatility", "rate", "dividend_yield", "option_type"):
+            rec[f] = getattr(self, f)
+        return rec
+
+    def take(self, idx) -> "OptionBatch":
+        return OptionBatch(*(getattr(self, f)[idx] for f in
+                             ("spot", "strike", "maturity", "volatility", "rate", "dividend_yield", "option_type")))
+
+
+def c1_puts(n: int = 64, seed: int = 0) -> OptionBatch:
+    """Plain configuration: American puts, S=100, K~U[80,120], T~U[0.1,2],
+    sigma~U[0.1,0.5], r~U[0,0.08], q~U[0,0.04] (draw order K,T,sigma,r,q)."""
+    g = Xoshiro256ss(seed)
+    cols = np.empty((n, 5))
+    for i in range(n):
+        cols[i, 0] = 80.0 + 40.0 * g.uniform()
+        cols[i, 1] = 0.1 + 1.9 * g.uniform()
+        cols[i, 2] = 0.10 + 0.40 * g.uniform()
+        cols[i, 3] = 0.08 * g.uniform()
+        cols[i, 4] = 0.04 * g.uniform()
+    return OptionBatch(np.full(n, 100.0), cols[:, 0].copy(), cols[:, 1].copy(), cols[:, 2].copy(),
+                       cols[:, 3].copy(), cols[:, 4].copy(), np.full(n, PUT, dtype=np.int32))
+
+
+def c2_chains(n_chains: int = 64, n_strikes: int = 64, seed: int = 1):
+    """Shared-grid chains: per chain S=100, T~U[0.05,2], sigma~U[0.1,0.6],
+    r~U[0,0.08], q~U[0,0.04]; strikes K_j = 70 + 60 j/(n_strikes-1), puts
+    for even j and calls for odd j. Returns (batch, chain_id)."""
+    g = Xoshiro256ss(seed)
+    rows = []
+    chain_id = []
+    for c in range(n_chains):
+        T = 0.05 + 1.95 * g.uniform()
+        sig = 0.1 + 0.5 * g.uniform()
+        r = 0.08 * g.uniform()
+        q = 0.04 * g.uniform()
+        for j in range(n_strikes):
+            K = 70.0 + 60.0 * j / (n_strikes - 1)
+            rows.append((100.0, K, T, sig, r, q, PUT if j % 2 == 0 else CALL))
+            chain_id.append(c)
+    a = np.array(rows)
+    return (OptionBatch(a[:, 0].copy(), a[:, 1].copy(), a[:, 2].copy(), a[:, 3].copy(), a[:, 4].copy(),
+                        a[:, 5].copy(), a[:, 6].astype(np.int32)), np.array(chain_id, dtype=np.int32))
+
+
+def c3_uneven(n: int = 65536, seed: int = 2) -> OptionBatch:
+    """Uneven batch: T = exp(U[ln(1/365), ln 3]), K/S~U[0.6,1.4] (S=100),
+    sigma~U[0.05,0.8], r~U[0,0.1], q~U[0,0.05]; types alternate put/call."""
+    g = Xoshiro256ss(seed)
+    a = np.empty((n, 5))
+    lo, hi = math.log(1.0 / 365.0), math.log(3.0)
+    for i in range(n):
+        a[i, 0] = math.exp(lo + (hi - lo) * g.uniform())
+        a[i, 1] = 100.0 * (0.6 + 0.8 * g.uniform())
+        a[i, 2] = 0.05 + 0.75 * g.uniform()
+        a[i, 3] = 0.1 * g.uniform()
+        a[i, 4] = 0.05 * g.uniform()
+    types = np.where(np.arange(n) % 2 == 0, PUT, CALL).astype(np.int32)
+    return OptionBatch(np.full(n, 100.0), a[:, 1].copy(), a[:, 0].copy(), a[:, 2].copy(), a[:, 3].copy(),
+                       a[:, 4].copy(), types)
+
+
+def replicate(batch: OptionBatch, n: int) -> OptionBatch:
+    """Tile a batch to n options (throughput variants of a configuration)."""
+    idx = np.arange(n) % len(batch)
+    return batch.take(idx)

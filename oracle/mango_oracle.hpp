@@ -1,0 +1,944 @@
+// SPDX-License-Identifier: MIT
+//
+// ORACLE — test infrastructure only.
+//
+// A CPU restatement, in C++20, of the reference's American-option PDE path
+// (accelas/mango-option). It exists to check the GPU library: only tests/,
+// __graft_entry__.smoke() and bench.py's CPU-baseline leg may load it. The
+// product library never links it and never falls back to it.
+//
+// Every function follows a reference file:line, with the same operation order
+// and the same std::fma placement. Build with -ffp-contract=off so that no
+// extra contraction changes rounding (the reference leaves contraction to the
+// compiler; its results differ at ULP level between ISAs).
+//
+// Pins (what checks each part, besides the GPU comparison):
+//   thomas / thomas_projected : reference KATs (tests/tridiagonal_solver_test.cc),
+//                               dense LU and LCP active-set brute force
+//   stencil / jacobian        : jacobian == stencil at 1e-14
+//                               (tests/internal/spatial_operator_jacobian_test.cc),
+//                               exactness on quadratics
+//   grid generation           : closed forms of uniform/sinh, monotonicity,
+//                               cluster merge rule
+//   spline                    : linear/cubic reproduction (tests/thomas_cubic_spline_test.cc)
+//   full price                : QuantLib golden constants at 0.1% / $0.01
+//                               (benchmarks/readme_benchmarks.cc:26-48), the
+//                               invariants of tests/american_option_test.cc,
+//                               no-dividend call == European, r=0 put == European,
+//                               second-order self-convergence
+//   normalized chain          : bitwise equality with per-option solves on the
+//                               same grid (strike-free PDE)
+//   parity unpinned           : the exact 1e-9-level trajectory of the deep-ITM
+//                               lock on sinh grids (no reference vector exists).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <limits>
+#include <optional>
+#include <utility>
+#include <vector>
+
+namespace oracle {
+
+enum class OptionType { CALL = 0, PUT = 1 };
+
+// PricingParams restricted to a constant rate and no discrete dividends
+// (src/option/option_spec.hpp:185-192, 262-281).
+struct PricingParams {
+    double spot = 0.0;
+    double strike = 0.0;
+    double maturity = 0.0;
+    double rate = 0.0;
+    double dividend_yield = 0.0;
+    double volatility = 0.0;
+    OptionType type = OptionType::PUT;
+};
+
+// validate_option_spec + validate_pricing_params (src/option/option_spec.cpp:8-50, 117-155).
+inline bool validate_pricing_params(const PricingParams& p) {
+    if (p.spot <= 0.0 || !std::isfinite(p.spot)) return false;
+    if (p.strike <= 0.0 || !std::isfinite(p.strike)) return false;
+    if (p.maturity <= 0.0 || !std::isfinite(p.maturity)) return false;
+    if (!std::isfinite(p.rate)) return false;
+    if (p.dividend_yield < 0.0 || !std::isfinite(p.dividend_yield)) return false;
+    if (p.volatility <= 0.0 || !std::isfinite(p.volatility)) return false;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Grid specification and generation (src/pde/core/grid.hpp:24-498)
+// ---------------------------------------------------------------------------
+struct Cluster {
+    double center_x;
+    double alpha;
+    double weight;
+};
+
+struct GridSpec {
+    enum class Type { Uniform = 0, LogSpaced = 1, SinhSpaced = 2, MultiSinhSpaced = 3 };
+    Type type = Type::SinhSpaced;
+    double x_min = -3.0;
+    double x_max = 3.0;
+    size_t n_points = 101;
+    double concentration = 1.0;
+    std::vector<Cluster> clusters;
+
+    // grid.hpp:55-67
+    static std::optional<GridSpec> uniform(double x_min, double x_max, size_t n) {
+        if (n < 2 || x_min >= x_max) return std::nullopt;
+        return GridSpec{Type::Uniform, x_min, x_max, n, 1.0, {}};
+    }
+    // grid.hpp:93-109
+    static std::optional<GridSpec> sinh_spaced(double x_min, double x_max, size_t n,
+                                               double concentration = 1.0) {
+        if (n < 2 || x_min >= x_max || concentration <= 0) return std::nullopt;
+        return GridSpec{Type::SinhSpaced, x_min, x_max, n, concentration, {}};
+    }
+    // grid.hpp:111-165
+    static std::optional<GridSpec> multi_sinh_spaced(double x_min, double x_max, size_t n,
+                                                     std::vector<Cluster> clusters,
+                                                     bool auto_merge = true) {
+        if (n < 2 || x_min >= x_max || clusters.empty()) return std::nullopt;
+        for (const auto& c : clusters) {
+            if (c.alpha <= 0 || c.weight <= 0) return std::nullopt;
+            if (c.center_x < x_min || c.center_x > x_max) return std::nullopt;
+        }
+        if (auto_merge) merge_nearby_clusters(clusters);
+        return GridSpec{Type::MultiSinhSpaced, x_min, x_max, n, 1.0, std::move(clusters)};
+    }
+
+    // grid.hpp:191-235
+    static void merge_nearby_clusters(std::vector<Cluster>& clusters) {
+        if (clusters.size() <= 1) return;
+        bool merged = true;
+        while (merged) {
+            merged = false;
+            for (size_t i = 0; i < clusters.size() && !merged; ++i) {
+                for (size_t j = i + 1; j < clusters.size() && !merged; ++j) {
+                    const double delta_x = std::abs(clusters[i].center_x - clusters[j].center_x);
+                    const double alpha_avg = (clusters[i].alpha + clusters[j].alpha) / 2.0;
+                    const double threshold = 0.3 / alpha_avg;
+                    const double epsilon = 100.0 * std::numeric_limits<double>::epsilon();
+                    if (delta_x <= threshold + epsilon) {
+                        const double total_weight = clusters[i].weight + clusters[j].weight;
+                        const double w_i = clusters[i].weight / total_weight;
+                        const double w_j = clusters[j].weight / total_weight;
+                        Cluster m{w_i * clusters[i].center_x + w_j * clusters[j].center_x,
+                                  w_i * clusters[i].alpha + w_j * clusters[j].alpha,
+                                  total_weight};
+                        clusters[i] = m;
+                        clusters.erase(clusters.begin() + static_cast<std::ptrdiff_t>(j));
+                        merged = true;
+                        break;
+                    }
+                }
+            }
+        }
+    }
+
+    // grid.hpp:237-262
+    static void enforce_monotonicity(std::vector<double>& points, double x_min, double x_max) {
+        const size_t n = points.size();
+        if (n < 2) return;
+        const double min_spacing = (x_max - x_min) / static_cast<double>(n * 100);
+        points[0] = x_min;
+        points[n - 1] = x_max;
+        for (size_t i = 1; i < n - 1; ++i) {
+            if (points[i] <= points[i - 1] + min_spacing) points[i] = points[i - 1] + min_spacing;
+        }
+        for (size_t i = n - 2; i > 0; --i) {
+            if (points[i] >= points[i + 1] - min_spacing) points[i] = points[i + 1] - min_spacing;
+        }
+    }
+
+    // grid.hpp:364-498
+    std::vector<double> generate() const {
+        std::vector<double> points;
+        points.reserve(n_points);
+        const size_t n = n_points;
+        switch (type) {
+            case Type::Uniform: {
+                const double dx = (x_max - x_min) / static_cast<double>(n - 1);
+                for (size_t i = 0; i < n; ++i) points.push_back(x_min + static_cast<double>(i) * dx);
+                break;
+            }
+            case Type::LogSpaced: {
+                const double log_min = std::log(x_min);
+                const double log_max = std::log(x_max);
+                const double d_log = (log_max - log_min) / static_cast<double>(n - 1);
+                for (size_t i = 0; i < n; ++i)
+                    points.push_back(std::exp(log_min + static_cast<double>(i) * d_log));
+                break;
+            }
+            case Type::SinhSpaced: {
+                const double c = concentration;
+                const double sinh_half_c = std::sinh(c / 2.0);
+                for (size_t i = 0; i < n; ++i) {
+                    const double eta = static_cast<double>(i) / static_cast<double>(n - 1);
+                    const double sinh_term = std::sinh(c * (eta - 0.5)) / sinh_half_c;
+                    const double normalized = (1.0 + sinh_term) / 2.0;
+                    points.push_back(x_min + (x_max - x_min) * normalized);
+                }
+                break;
+            }
+            case Type::MultiSinhSpaced: {
+                const double range = x_max - x_min;
+                if (clusters.size() == 1) {
+                    const auto& cl = clusters[0];
+                    const double c = cl.alpha;
+                    const double center = cl.center_x;
+                    const double sinh_half_c = std::sinh(c / 2.0);
+                    const double eta_center = (center - x_min) / range;
+                    const bool is_centered = std::abs(eta_center - 0.5) < 1e-10;
+                    if (is_centered) {
+                        for (size_t i = 0; i < n; ++i) {
+                            const double eta = static_cast<double>(i) / static_cast<double>(n - 1);
+                            const double sinh_term = std::sinh(c * (eta - 0.5)) / sinh_half_c;
+                            const double normalized = (1.0 + sinh_term) / 2.0;
+                            points.push_back(x_min + range * normalized);
+                        }
+                    } else {
+                        const double offset = center - (x_min + x_max) / 2.0;
+                        std::vector<double> raw(n);
+                        for (size_t i = 0; i < n; ++i) {
+                            const double eta = static_cast<double>(i) / static_cast<double>(n - 1);
+                            const double sinh_term = std::sinh(c * (eta - eta_center)) / sinh_half_c;
+                            const double normalized = (1.0 + sinh_term) / 2.0;
+                            raw[i] = x_min + range * normalized + offset;
+                        }
+                        enforce_monotonicity(raw, x_min, x_max);
+                        points = std::move(raw);
+                    }
+                } else {
+                    std::vector<double> raw(n);
+                    double total_weight = 0.0;
+                    for (const auto& cl : clusters) total_weight += cl.weight;
+                    for (size_t i = 0; i < n; ++i) {
+                        const double eta = static_cast<double>(i) / static_cast<double>(n - 1);
+                        double weighted_x = 0.0;
+                        for (const auto& cl : clusters) {
+                            const double c = cl.alpha;
+                            const double center = cl.center_x;
+                            const double w = cl.weight / total_weight;
+                            const double sinh_half_c = std::sinh(c / 2.0);
+                            const double eta_center = (center - x_min) / range;
+                            const double sinh_term = std::sinh(c * (eta - eta_center)) / sinh_half_c;
+                            const double normalized = (1.0 + sinh_term) / 2.0;
+                            const double offset = center - (x_min + x_max) / 2.0;
+                            const double x_i = x_min + range * normalized + offset;
+                            weighted_x += w * x_i;
+                        }
+                        raw[i] = weighted_x;
+                    }
+                    enforce_monotonicity(raw, x_min, x_max);
+                    points = std::move(raw);
+                }
+                break;
+            }
+        }
+        return points;
+    }
+};
+
+// GridSpacing::compute_spacing uniformity test (grid.hpp:665-699).
+inline bool grid_is_uniform(const std::vector<double>& x) {
+    const size_t n = x.size();
+    if (n < 2) return true;
+    const double expected_dx = (x[n - 1] - x[0]) / static_cast<double>(n - 1);
+    const double tolerance = 100.0 * std::numeric_limits<double>::epsilon();
+    for (size_t i = 1; i < n; ++i) {
+        const double actual_dx = x[i] - x[i - 1];
+        if (std::abs(actual_dx - expected_dx) > tolerance) return false;
+    }
+    return true;
+}
+
+// GridSpacing: uniform constants or the 5 non-uniform arrays (grid.hpp:513-614).
+struct Spacing {
+    bool uniform = false;
+    double dx = 0.0, dx_inv = 0.0, dx_inv_sq = 0.0;
+    std::vector<double> dxl_inv, dxr_inv, dxc_inv, w_left, w_right;  // size n-2
+
+    explicit Spacing(const std::vector<double>& x) {
+        const size_t n = x.size();
+        uniform = grid_is_uniform(x);
+        if (uniform) {
+            dx = (x[n - 1] - x[0]) / static_cast<double>(n - 1);
+            dx_inv = 1.0 / dx;
+            dx_inv_sq = dx_inv * dx_inv;
+            return;
+        }
+        const size_t interior = n - 2;
+        dxl_inv.resize(interior); dxr_inv.resize(interior); dxc_inv.resize(interior);
+        w_left.resize(interior); w_right.resize(interior);
+        for (size_t i = 1; i <= n - 2; ++i) {
+            const double dx_left = x[i] - x[i - 1];
+            const double dx_right = x[i + 1] - x[i];
+            const double dx_center = 0.5 * (dx_left + dx_right);
+            const size_t idx = i - 1;
+            dxl_inv[idx] = 1.0 / dx_left;
+            dxr_inv[idx] = 1.0 / dx_right;
+            dxc_inv[idx] = 1.0 / dx_center;
+            w_left[idx] = dx_right / (dx_left + dx_right);
+            w_right[idx] = dx_left / (dx_left + dx_right);
+        }
+    }
+
+    // CenteredDifference (src/pde/operators/centered_difference.hpp:26-99)
+    void second_derivative(const double* u, double* d2, size_t start, size_t end) const {
+        if (uniform) {
+            const double dx2_inv = dx_inv_sq;
+            for (size_t i = start; i < end; ++i)
+                d2[i] = std::fma(u[i + 1] + u[i - 1], dx2_inv, -2.0 * u[i] * dx2_inv);
+        } else {
+            for (size_t i = start; i < end; ++i) {
+                const double fwd = (u[i + 1] - u[i]) * dxr_inv[i - 1];
+                const double bwd = (u[i] - u[i - 1]) * dxl_inv[i - 1];
+                d2[i] = (fwd - bwd) * dxc_inv[i - 1];
+            }
+        }
+    }
+    void first_derivative(const double* u, double* d1, size_t start, size_t end) const {
+        if (uniform) {
+            const double half_dx_inv = dx_inv * 0.5;
+            for (size_t i = start; i < end; ++i) d1[i] = (u[i + 1] - u[i - 1]) * half_dx_inv;
+        } else {
+            for (size_t i = start; i < end; ++i) {
+                const double term1 = w_left[i - 1] * (u[i] - u[i - 1]) * dxl_inv[i - 1];
+                const double term2 = w_right[i - 1] * (u[i + 1] - u[i]) * dxr_inv[i - 1];
+                d1[i] = term1 + term2;
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Time domain (src/pde/core/time_domain.hpp:36-151), uniform steps only
+// ---------------------------------------------------------------------------
+struct TimeDomain {
+    double t_start = 0.0, t_end = 1.0, dt = 0.01;
+    size_t n_steps = 100;
+    static TimeDomain from_n_steps(double t0, double t1, size_t n) {
+        TimeDomain td;
+        td.t_start = t0; td.t_end = t1; td.n_steps = n;
+        td.dt = (t1 - t0) / static_cast<double>(n);
+        return td;
+    }
+    double time_point(size_t i) const { return t_start + static_cast<double>(i) * dt; }
+};
+
+// ---------------------------------------------------------------------------
+// Grid estimation (src/option/grid_spec_types.{hpp,cpp})
+// ---------------------------------------------------------------------------
+inline double optimal_sinh_alpha(double n_sigma) { return 2.0 * std::asinh(n_sigma / std::sqrt(2.0)); }
+
+struct GridAccuracyParams {
+    double n_sigma = 5.0;
+    double alpha = optimal_sinh_alpha(5.0);
+    double tol = 1e-2;
+    double c_t = 0.75;
+    size_t min_spatial_points = 100;
+    size_t max_spatial_points = 1200;
+    size_t max_time_steps = 5000;
+};
+
+// grid_spec_types.cpp:37-118 (no discrete dividends)
+inline std::pair<GridSpec, TimeDomain> estimate_pde_grid(const PricingParams& p,
+                                                         const GridAccuracyParams& acc = {}) {
+    const double sigma_sqrt_T = p.volatility * std::sqrt(p.maturity);
+    const double x0 = std::log(p.spot / p.strike);
+    const double x_min = x0 - acc.n_sigma * sigma_sqrt_T;
+    const double x_max = x0 + acc.n_sigma * sigma_sqrt_T;
+    const double dx_target = p.volatility * std::sqrt(acc.tol);
+    size_t Nx = static_cast<size_t>(std::ceil((x_max - x_min) / dx_target));
+    Nx = std::clamp(Nx, acc.min_spatial_points, acc.max_spatial_points);
+    if (Nx % 2 == 0) Nx++;
+    std::vector<Cluster> clusters;
+    clusters.push_back({x0, acc.alpha, 0.5});
+    if (0.0 >= x_min && 0.0 <= x_max) clusters.push_back({0.0, acc.alpha, 1.0});
+    auto spec = GridSpec::multi_sinh_spaced(x_min, x_max, Nx, std::move(clusters));
+    if (!spec) spec = GridSpec::sinh_spaced(x_min, x_max, Nx, acc.alpha);
+    const auto pts = spec->generate();
+    double dx_min = pts[1] - pts[0];
+    for (size_t i = 2; i < pts.size(); ++i) dx_min = std::min(dx_min, pts[i] - pts[i - 1]);
+    const double dt = acc.c_t * dx_min;
+    size_t Nt = static_cast<size_t>(std::ceil(p.maturity / dt));
+    Nt = std::min(Nt, acc.max_time_steps);
+    return {*spec, TimeDomain::from_n_steps(0.0, p.maturity, Nt)};
+}
+
+// grid_spec_types.cpp:120-177 (no discrete dividends)
+inline std::pair<GridSpec, TimeDomain> estimate_batch_pde_grid(const std::vector<PricingParams>& ps,
+                                                               const GridAccuracyParams& acc = {}) {
+    if (ps.empty()) {
+        return {*GridSpec::sinh_spaced(-3.0, 3.0, 101, acc.alpha), TimeDomain::from_n_steps(0.0, 1.0, 100)};
+    }
+    double gx_min = std::numeric_limits<double>::max();
+    double gx_max = std::numeric_limits<double>::lowest();
+    size_t gNx = 0, gNt = 0;
+    double max_T = 0.0;
+    for (const auto& p : ps) {
+        auto [s, td] = estimate_pde_grid(p, acc);
+        gx_min = std::min(gx_min, s.x_min);
+        gx_max = std::max(gx_max, s.x_max);
+        gNx = std::max(gNx, s.n_points);
+        gNt = std::max(gNt, td.n_steps);
+        max_T = std::max(max_T, p.maturity);
+    }
+    auto spec = GridSpec::multi_sinh_spaced(gx_min, gx_max, gNx, {{0.0, acc.alpha, 1.0}});
+    if (!spec) spec = GridSpec::sinh_spaced(gx_min, gx_max, gNx, acc.alpha);
+    return {*spec, TimeDomain::from_n_steps(0.0, max_T, gNt)};
+}
+
+// ---------------------------------------------------------------------------
+// Thomas solvers (src/math/thomas_solver.hpp:86-198, 321-420)
+// ---------------------------------------------------------------------------
+constexpr double kSingularTol = 1e-15;
+
+inline bool solve_thomas(const double* lower, const double* diag, const double* upper,
+                         const double* rhs, double* sol, size_t n, double* ws) {
+    if (n == 0) return true;
+    if (n == 1) {
+        if (std::abs(diag[0]) < kSingularTol) return false;
+        sol[0] = rhs[0] / diag[0];
+        return true;
+    }
+    double* cp = ws;
+    double* dp = ws + n;
+    if (std::abs(diag[0]) < kSingularTol) return false;
+    cp[0] = upper[0] / diag[0];
+    dp[0] = rhs[0] / diag[0];
+    for (size_t i = 1; i < n - 1; ++i) {
+        const double denom = std::fma(-lower[i - 1], cp[i - 1], diag[i]);
+        if (std::abs(denom) < kSingularTol) return false;
+        const double inv = 1.0 / denom;
+        cp[i] = upper[i] * inv;
+        dp[i] = std::fma(-lower[i - 1], dp[i - 1], rhs[i]) * inv;
+    }
+    {
+        const size_t i = n - 1;
+        const double denom = std::fma(-lower[i - 1], cp[i - 1], diag[i]);
+        if (std::abs(denom) < kSingularTol) return false;
+        dp[i] = std::fma(-lower[i - 1], dp[i - 1], rhs[i]) / denom;
+    }
+    sol[n - 1] = dp[n - 1];
+    for (size_t i = n - 1; i > 0; --i) sol[i - 1] = std::fma(-cp[i - 1], sol[i], dp[i - 1]);
+    return true;
+}
+
+inline bool solve_thomas_projected(const double* lower, const double* diag, const double* upper,
+                                   const double* rhs, const double* psi, double* sol, size_t n,
+                                   double* ws) {
+    if (n == 0) return true;
+    if (n == 1) {
+        if (std::abs(diag[0]) < kSingularTol) return false;
+        sol[0] = std::max(rhs[0] / diag[0], psi[0]);
+        return true;
+    }
+    double* cp = ws;
+    double* dp = ws + n;
+    if (std::abs(diag[0]) < kSingularTol) return false;
+    cp[0] = upper[0] / diag[0];
+    dp[0] = rhs[0] / diag[0];
+    for (size_t i = 1; i < n - 1; ++i) {
+        const double denom = std::fma(-lower[i - 1], cp[i - 1], diag[i]);
+        if (std::abs(denom) < kSingularTol) return false;
+        const double inv = 1.0 / denom;
+        cp[i] = upper[i] * inv;
+        dp[i] = std::fma(-lower[i - 1], dp[i - 1], rhs[i]) * inv;
+    }
+    {
+        const size_t i = n - 1;
+        const double denom = std::fma(-lower[i - 1], cp[i - 1], diag[i]);
+        if (std::abs(denom) < kSingularTol) return false;
+        dp[i] = std::fma(-lower[i - 1], dp[i - 1], rhs[i]) / denom;
+    }
+    sol[n - 1] = std::max(dp[n - 1], psi[n - 1]);
+    for (size_t i = n - 1; i > 0; --i) {
+        const double unconstrained = std::fma(-cp[i - 1], sol[i], dp[i - 1]);
+        sol[i - 1] = std::max(unconstrained, psi[i - 1]);
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Natural cubic spline (src/math/cubic_spline_solver.hpp:74-370)
+// ---------------------------------------------------------------------------
+struct CubicSpline {
+    std::vector<double> x, y, a, b, c, d, h;
+
+    bool build(const double* xs, const double* ys, size_t n) {
+        if (n < 2) return false;
+        for (size_t i = 1; i < n; ++i) if (xs[i] <= xs[i - 1]) return false;
+        x.assign(xs, xs + n);
+        y.assign(ys, ys + n);
+        const size_t ni = n - 1;
+        a.resize(ni); b.resize(ni); c.assign(n, 0.0); d.resize(ni); h.resize(ni);
+        for (size_t i = 0; i < ni; ++i) h[i] = x[i + 1] - x[i];
+        c[0] = 0; c[n - 1] = 0;
+        if (n > 2) {
+            const size_t m = n - 2;
+            std::vector<double> lo(m > 0 ? m - 1 : 0), dg(m), up(m > 0 ? m - 1 : 0), rhs(m), sol(m), ws(2 * m);
+            for (size_t i = 0; i < m; ++i) {
+                const size_t k = i + 1;
+                dg[i] = 2.0 * (h[k - 1] + h[k]);
+                if (i > 0) lo[i - 1] = h[k - 1];
+                if (i < m - 1) up[i] = h[k];
+                const double slope_right = (y[k + 1] - y[k]) / h[k];
+                const double slope_left = (y[k] - y[k - 1]) / h[k - 1];
+                rhs[i] = 3.0 * (slope_right - slope_left);
+            }
+            if (!solve_thomas(lo.data(), dg.data(), up.data(), rhs.data(), sol.data(), m, ws.data()))
+                return false;
+            for (size_t i = 0; i < m; ++i) c[i + 1] = sol[i];
+        }
+        for (size_t i = 0; i < ni; ++i) {
+            a[i] = y[i];
+            b[i] = (y[i + 1] - y[i]) / h[i] - h[i] * (2.0 * c[i] + c[i + 1]) / 3.0;
+            d[i] = (c[i + 1] - c[i]) / (3.0 * h[i]);
+        }
+        return true;
+    }
+    size_t find_interval(double xe) const {
+        if (xe <= x[0]) return 0;
+        if (xe >= x.back()) return x.size() - 2;
+        auto it = std::lower_bound(x.begin(), x.end(), xe);
+        size_t idx = static_cast<size_t>(std::distance(x.begin(), it));
+        if (idx > 0 && (idx == x.size() || x[idx] > xe)) --idx;
+        return std::min(idx, x.size() - 2);
+    }
+    double eval(double xe) const {
+        if (x.empty()) return 0.0;
+        if (x.size() == 1) return y[0];
+        const size_t i = find_interval(xe);
+        const double dx = xe - x[i];
+        return a[i] + b[i] * dx + c[i] * (dx * dx) + d[i] * (dx * dx * dx);
+    }
+    double eval_derivative(double xe) const {
+        if (x.size() < 2) return 0.0;
+        const size_t i = find_interval(xe);
+        const double dx = xe - x[i];
+        return b[i] + 2.0 * c[i] * dx + 3.0 * d[i] * (dx * dx);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// PDE solve: American put/call, constant rate
+// (src/pde/internal/pde_solver.hpp:96-729, 865-972;
+//  src/option/american_option.cpp:150-312, 354-438)
+// ---------------------------------------------------------------------------
+enum Status : int32_t { kOk = 0, kLinearSolveFailure = 1, kInvalidConfiguration = 2, kValidation = 3 };
+
+struct TRBDF2 {
+    double gamma = 2.0 - std::sqrt(2.0);  // trbdf2_config.hpp:20-49
+    bool rannacher_startup = true;
+};
+
+struct Solution {
+    std::vector<double> x, u, u_prev;
+    double dt = 0.0;
+    size_t n_steps = 0;
+    bool uniform = false;
+    // snapshots (grid.hpp:745-995): state indices and rows
+    std::vector<size_t> snapshot_indices;
+    std::vector<double> snapshot_times;
+    std::vector<double> snapshots;  // [n_snap][n]
+};
+
+struct SolveOptions {
+    TRBDF2 trbdf2;
+    std::vector<double> snapshot_times;  // time-to-expiry values to record
+};
+
+// convert_times_to_indices, uniform branch (src/pde/core/grid.hpp:754-836)
+inline bool convert_times_to_indices(const std::vector<double>& times, const TimeDomain& td,
+                                     std::vector<size_t>& idx_out, std::vector<double>& t_out) {
+    if (td.n_steps == 0 || td.dt <= 0.0) return false;
+    std::vector<std::pair<size_t, double>> paired;
+    for (double t : times) {
+        if (t < td.t_start || t > td.t_end) return false;
+        const double step_exact = (t - td.t_start) / td.dt;
+        size_t si = static_cast<size_t>(std::floor(step_exact + 0.5));
+        si = std::min(si, td.n_steps);
+        paired.push_back({si, td.t_start + static_cast<double>(si) * td.dt});
+    }
+    std::sort(paired.begin(), paired.end());
+    paired.erase(std::unique(paired.begin(), paired.end()), paired.end());
+    idx_out.clear(); t_out.clear();
+    for (auto& [i, t] : paired) { idx_out.push_back(i); t_out.push_back(t); }
+    return true;
+}
+
+// Solve one American option on an explicit grid.
+// AmericanOptionSolver::solve (src/option/american_option.cpp:354-438) driving
+// PDESolver::initialize/solve (src/pde/internal/pde_solver.hpp:96-178).
+inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, const TimeDomain& td,
+                              Solution& out, const SolveOptions& opts = {}) {
+    if (!validate_pricing_params(p)) return kValidation;
+    const std::vector<double> x = spec.generate();
+    const size_t n = x.size();
+    if (n < 3) return kInvalidConfiguration;
+    const Spacing sp(x);
+    out = Solution{};
+    out.x = x;
+    out.dt = td.dt;
+    out.n_steps = td.n_steps;
+    out.uniform = sp.uniform;
+    if (!opts.snapshot_times.empty()) {
+        if (!convert_times_to_indices(opts.snapshot_times, td, out.snapshot_indices, out.snapshot_times))
+            return kInvalidConfiguration;
+        out.snapshots.assign(out.snapshot_indices.size() * n, 0.0);
+    }
+    auto record = [&](size_t state_idx, const std::vector<double>& u) {
+        auto it = std::lower_bound(out.snapshot_indices.begin(), out.snapshot_indices.end(), state_idx);
+        if (it == out.snapshot_indices.end() || *it != state_idx) return;
+        const size_t k = static_cast<size_t>(it - out.snapshot_indices.begin());
+        std::copy(u.begin(), u.end(), out.snapshots.begin() + static_cast<std::ptrdiff_t>(k * n));
+    };
+
+    const bool put = p.type == OptionType::PUT;
+    const double r = p.rate;
+    const double q = p.dividend_yield;
+    // BlackScholesPDE (src/pde/operators/black_scholes_pde.hpp:41-46, 72-76, 84-104)
+    const double hss = 0.5 * p.volatility * p.volatility;
+    auto pde = [&](double d2, double d1, double v) {
+        const double drift = r - q - hss;
+        return hss * d2 + drift * d1 - r * v;
+    };
+    // Payoff / obstacle (src/option/option_spec.hpp:121-145)
+    std::vector<double> psi(n);
+    for (size_t i = 0; i < n; ++i)
+        psi[i] = put ? std::max(1.0 - std::exp(x[i]), 0.0) : std::max(std::exp(x[i]) - 1.0, 0.0);
+    // Dirichlet BCs (src/option/american_option.cpp:192-202, 276-288)
+    auto g_left = [&](double) { return put ? std::max(1.0 - std::exp(x[0]), 0.0) : 0.0; };
+    auto g_right = [&](double t) { return put ? 0.0 : std::exp(x[n - 1]) - std::exp(-r * t); };
+    auto apply_bc = [&](std::vector<double>& u, double t) { u[0] = g_left(t); u[n - 1] = g_right(t); };
+
+    std::vector<double> d2(n, 0.0), d1(n, 0.0);
+    // apply_spatial_operator (pde_solver.hpp:322-335; spatial_operator.hpp:74-98)
+    auto apply_L = [&](const std::vector<double>& u, std::vector<double>& Lu) {
+        std::fill(d2.begin() + 1, d2.begin() + static_cast<std::ptrdiff_t>(n - 1), 0.0);
+        std::fill(d1.begin() + 1, d1.begin() + static_cast<std::ptrdiff_t>(n - 1), 0.0);
+        sp.second_derivative(u.data(), d2.data(), 1, n - 1);
+        sp.first_derivative(u.data(), d1.data(), 1, n - 1);
+        for (size_t i = 1; i < n - 1; ++i) Lu[i] = pde(d2[i], d1[i], u[i]);
+        Lu[0] = Lu[n - 1] = 0.0;
+    };
+    // Analytic Jacobian bands of L (spatial_operator.hpp:123-164); constant
+    // for a constant rate, so evaluated once (bitwise what the reference
+    // recomputes each stage).
+    std::vector<double> jl(n, 0.0), jd(n, 0.0), ju(n, 0.0);
+    {
+        const double a = hss;
+        const double b = r - q - hss;
+        const double c = -r;
+        for (size_t i = 1; i < n - 1; ++i) {
+            const double dx_left = x[i] - x[i - 1];
+            const double dx_right = x[i + 1] - x[i];
+            const double dx_avg = (dx_left + dx_right) / 2.0;
+            const double d2_im1 = a / (dx_left * dx_avg);
+            const double d2_i = -a * (1.0 / dx_left + 1.0 / dx_right) / dx_avg;
+            const double d2_ip1 = a / (dx_right * dx_avg);
+            const double d1_denom = dx_left + dx_right;
+            const double d1_im1 = -b * dx_right / (dx_left * d1_denom);
+            const double d1_i = b * (dx_right - dx_left) / (dx_left * dx_right);
+            const double d1_ip1 = b * dx_left / (dx_right * d1_denom);
+            jl[i] = d2_im1 + d1_im1;
+            jd[i] = d2_i + d1_i + c;
+            ju[i] = d2_ip1 + d1_ip1;
+        }
+    }
+    std::vector<double> lower(n - 1), diag(n), upper(n - 1), rhs_bc(n), lpsi(n), ws(2 * n);
+    double psi_max = *std::max_element(psi.begin(), psi.end());
+
+    // solve_implicit_stage_projected (pde_solver.hpp:525-729)
+    auto stage = [&](double t, double w, std::vector<double>& u, const std::vector<double>& rhs) -> bool {
+        apply_bc(u, t);
+        for (size_t i = 1; i < n - 1; ++i) {
+            lower[i - 1] = -w * jl[i];
+            diag[i] = 1.0 - w * jd[i];
+            upper[i] = -w * ju[i];
+        }
+        diag[0] = 1.0; upper[0] = 0.0;          // build_jacobian_boundaries (:924-972)
+        diag[n - 1] = 1.0; lower[n - 2] = 0.0;
+        rhs_bc = rhs;
+        rhs_bc[0] = g_left(t);
+        rhs_bc[n - 1] = g_right(t);
+        if (psi_max > 0.0) {
+            const double thr = 0.95 * psi_max;
+            size_t first = n - 1;
+            for (size_t i = 1; i < n - 1; ++i) {
+                if (psi[i] > thr && u[i] - psi[i] < 1e-8) { first = i; break; }
+            }
+            if (first < n - 1) {
+                apply_L(psi, lpsi);
+                for (size_t i = first; i < n - 1; ++i) {
+                    if (psi[i] > thr && u[i] - psi[i] < 1e-8 && lpsi[i] < 0.0) {
+                        lower[i - 1] = 0.0; diag[i] = 1.0; upper[i] = 0.0; rhs_bc[i] = psi[i];
+                    }
+                }
+            }
+        }
+        if (!solve_thomas_projected(lower.data(), diag.data(), upper.data(), rhs_bc.data(),
+                                    psi.data(), u.data(), n, ws.data()))
+            return false;
+        apply_bc(u, t);
+        return true;
+    };
+
+    std::vector<double> u(n), u_prev(n), lu(n), rhs(n);
+    // initialize (pde_solver.hpp:96-109)
+    u = psi;
+    apply_bc(u, td.t_start);
+    for (size_t i = 0; i < n; ++i) u[i] = std::max(u[i], psi[i]);
+    u_prev = u;
+
+    const double gamma = opts.trbdf2.gamma;
+    double t = td.time_point(0);
+    record(0, u);
+    size_t step = 0;
+    if (opts.trbdf2.rannacher_startup && td.n_steps > 0) {
+        // solve_rannacher_startup (pde_solver.hpp:413-447)
+        const double dt = td.dt;
+        const double t_next = t + dt;
+        const double half_dt = 0.5 * dt;
+        const double t_mid = t + half_dt;
+        u_prev = u;
+        rhs = u_prev; u = u_prev;
+        if (!stage(t_mid, half_dt, u, rhs)) return kLinearSolveFailure;
+        u_prev = u;
+        rhs = u_prev; u = u_prev;
+        if (!stage(t_next, half_dt, u, rhs)) return kLinearSolveFailure;
+        t = td.time_point(1);
+        record(1, u);
+        step = 1;
+    }
+    for (; step < td.n_steps; ++step) {
+        const double dt = td.dt;
+        const double t_next = td.time_point(step + 1);
+        u_prev = u;
+        // stage 1 (pde_solver.hpp:342-362)
+        const double t_stage1 = t + gamma * dt;
+        const double w1 = gamma * dt / 2.0;
+        apply_L(u_prev, lu);
+        for (size_t i = 0; i < n; ++i) rhs[i] = std::fma(w1, lu[i], u_prev[i]);
+        u = u_prev;
+        if (!stage(t_stage1, w1, u, rhs)) return kLinearSolveFailure;
+        // stage 2 (pde_solver.hpp:370-395)
+        const double omg = 1.0 - gamma;
+        const double tmg = 2.0 - gamma;
+        const double den = gamma * tmg;
+        const double alpha = 1.0 / den;
+        const double beta = -(omg * omg) / den;
+        const double w2 = (1.0 - gamma) * dt / (2.0 - gamma);
+        for (size_t i = 0; i < n; ++i) rhs[i] = std::fma(alpha, u[i], beta * u_prev[i]);
+        if (!stage(t_next, w2, u, rhs)) return kLinearSolveFailure;
+        t = t_next;
+        record(step + 1, u);
+    }
+    out.u = u;
+    out.u_prev = u_prev;
+    return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// Greeks at spot (src/option/american_option_result.cpp:25-165)
+// ---------------------------------------------------------------------------
+struct Greeks {
+    double value = 0.0, delta = 0.0, gamma = 0.0, theta = 0.0;
+};
+
+inline double value_at(const Solution& s, double spot, double strike) {
+    const size_t n = s.x.size();
+    const double x = std::log(spot / strike);
+    if (x <= s.x[0]) return s.u[0] * strike;
+    if (x >= s.x[n - 1]) return s.u[n - 1] * strike;
+    CubicSpline sp;
+    sp.build(s.x.data(), s.u.data(), n);
+    return std::max(0.0, sp.eval(x)) * strike;
+}
+
+inline Greeks compute_greeks(const Solution& s, double spot, double strike) {
+    Greeks g;
+    const size_t n = s.x.size();
+    CubicSpline spl;
+    spl.build(s.x.data(), s.u.data(), n);
+    const double xs = std::log(spot / strike);
+    // value_at (:29-49)
+    if (xs <= s.x[0]) g.value = s.u[0] * strike;
+    else if (xs >= s.x[n - 1]) g.value = s.u[n - 1] * strike;
+    else g.value = std::max(0.0, spl.eval(xs)) * strike;
+    // delta (:51-60)
+    g.delta = spl.eval_derivative(xs) * (strike / spot);
+    // gamma (:62-98): stencil outputs on [1, n-2), the rest left at zero
+    {
+        const Spacing sp(s.x);
+        std::vector<double> dv(n, 0.0), d2v(n, 0.0);
+        sp.first_derivative(s.u.data(), dv.data(), 1, n - 2);
+        sp.second_derivative(s.u.data(), d2v.data(), 1, n - 2);
+        size_t il, ir;
+        // find_grid_index (:123-143)
+        if (xs <= s.x[0]) { il = ir = 0; }
+        else if (xs >= s.x[n - 1]) { il = ir = n - 1; }
+        else {
+            auto it = std::lower_bound(s.x.begin(), s.x.end(), xs);
+            ir = static_cast<size_t>(std::distance(s.x.begin(), it));
+            il = ir > 0 ? ir - 1 : 0;
+            if (std::abs(s.x[ir] - xs) < 1e-14) il = ir;
+        }
+        double dvs, d2vs;
+        if (il == ir) { dvs = dv[il]; d2vs = d2v[il]; }
+        else {
+            const double al = (xs - s.x[il]) / (s.x[ir] - s.x[il]);
+            dvs = dv[il] + al * (dv[ir] - dv[il]);
+            d2vs = d2v[il] + al * (d2v[ir] - d2v[il]);
+        }
+        const double k_over_s2 = strike / (spot * spot);
+        g.gamma = std::fma(k_over_s2, d2vs, -k_over_s2 * dvs);
+    }
+    // theta (:100-121)
+    {
+        const double v_cur = spl.eval(xs);
+        CubicSpline prev;
+        prev.build(s.x.data(), s.u_prev.data(), n);
+        const double v_prev = prev.eval(xs);
+        g.theta = (v_prev - v_cur) / s.dt * strike;
+    }
+    return g;
+}
+
+// ---------------------------------------------------------------------------
+// Batch / normalized chain (src/option/american_option_batch.cpp:26-518)
+// ---------------------------------------------------------------------------
+struct PDEGridConfig {
+    GridSpec spec;
+    size_t n_time = 1000;
+};
+
+struct BatchItem {
+    int32_t status = kInvalidConfiguration;
+    Solution sol;         // grid + solution the result reads (shared grid for chain members)
+    Greeks greeks;
+};
+
+struct BatchSolver {
+    GridAccuracyParams accuracy;
+    bool use_normalized = true;
+    SolveOptions opts;
+    static constexpr double MAX_WIDTH = 5.8;
+    static constexpr double MAX_DX = 0.05;
+    static constexpr double MIN_MARGIN_ABS = 0.35;
+    int n_threads = 1;
+    size_t pde_solves = 0;  // diagnostics: PDE solves actually run by the last call
+
+    // is_normalized_eligible (:26-107)
+    bool is_normalized_eligible(const std::vector<PricingParams>& ps, bool shared) const {
+        if (!shared || ps.empty()) return false;
+        for (size_t i = 1; i < ps.size(); ++i) if (ps[i].type != ps[0].type) return false;
+        for (const auto& p : ps) if (p.spot <= 0.0 || p.strike <= 0.0) return false;
+        auto [spec, td] = estimate_pde_grid(ps[0], accuracy);
+        (void)td;
+        const double dx = (spec.x_max - spec.x_min) / static_cast<double>(spec.n_points - 1);
+        if (dx > MAX_DX) return false;
+        if (spec.x_max - spec.x_min > MAX_WIDTH) return false;
+        double m_min = std::numeric_limits<double>::infinity(), m_max = -m_min;
+        for (const auto& p : ps) { const double m = p.spot / p.strike; m_min = std::min(m_min, m); m_max = std::max(m_max, m); }
+        const double margin_left = std::log(m_min) - spec.x_min;
+        const double margin_right = spec.x_max - std::log(m_max);
+        const double min_margin = std::max(MIN_MARGIN_ABS, 6.0 * dx);
+        return !(margin_left < min_margin || margin_right < min_margin);
+    }
+
+    // solve_regular_batch (:384-518)
+    std::vector<BatchItem> solve_regular(const std::vector<PricingParams>& ps, bool shared,
+                                         const std::optional<PDEGridConfig>& custom) {
+        std::vector<BatchItem> res(ps.size());
+        if (ps.empty()) return res;
+        std::optional<std::pair<GridSpec, TimeDomain>> shared_grid;
+        std::optional<std::pair<GridSpec, size_t>> resolved_custom;
+        if (custom) resolved_custom = std::make_pair(custom->spec, custom->n_time);
+        if (shared) {
+            if (resolved_custom)
+                shared_grid = std::make_pair(resolved_custom->first, TimeDomain::from_n_steps(0.0, ps[0].maturity, resolved_custom->second));
+            else
+                shared_grid = estimate_batch_pde_grid(ps, accuracy);
+        }
+        pde_solves += ps.size();
+        #pragma omp parallel for schedule(static) num_threads(n_threads)
+        for (size_t i = 0; i < ps.size(); ++i) {
+            GridSpec spec;
+            size_t nt;
+            if (shared) { spec = shared_grid->first; nt = shared_grid->second.n_steps; }
+            else if (resolved_custom) { spec = resolved_custom->first; nt = resolved_custom->second; }
+            else { auto g = estimate_pde_grid(ps[i], accuracy); spec = g.first; nt = g.second.n_steps; }
+            if (!validate_pricing_params(ps[i])) { res[i].status = kValidation; continue; }
+            const auto td = TimeDomain::from_n_steps(0.0, ps[i].maturity, nt);
+            res[i].status = solve_american(ps[i], spec, td, res[i].sol, opts);
+            if (res[i].status == kOk) res[i].greeks = compute_greeks(res[i].sol, ps[i].spot, ps[i].strike);
+        }
+        return res;
+    }
+
+    // group_by_pde_parameters (:219-270) + solve_normalized_chain (:313-382)
+    std::vector<BatchItem> solve_normalized_chain(const std::vector<PricingParams>& ps,
+                                                  const std::optional<PDEGridConfig>& custom) {
+        struct Group { PricingParams p; std::vector<size_t> idx; };
+        std::vector<Group> groups;
+        constexpr double TOL = 1e-10;
+        for (size_t i = 0; i < ps.size(); ++i) {
+            const auto& p = ps[i];
+            bool found = false;
+            for (auto& g : groups) {
+                if (std::abs(g.p.volatility - p.volatility) < TOL && std::abs(g.p.rate - p.rate) < TOL &&
+                    std::abs(g.p.dividend_yield - p.dividend_yield) < TOL &&
+                    std::abs(g.p.maturity - p.maturity) < TOL && g.p.type == p.type) {
+                    g.idx.push_back(i); found = true; break;
+                }
+            }
+            if (!found) {
+                PricingParams np{1.0, 1.0, p.maturity, p.rate, p.dividend_yield, p.volatility, p.type};
+                groups.push_back({np, {i}});
+            }
+        }
+        std::vector<BatchItem> res(ps.size());
+        for (auto& g : groups) {
+            auto one = solve_regular({g.p}, true, custom);
+            for (size_t idx : g.idx) {
+                res[idx].status = one[0].status;
+                if (one[0].status != kOk) continue;
+                res[idx].sol = one[0].sol;
+                res[idx].greeks = compute_greeks(one[0].sol, ps[idx].spot, ps[idx].strike);
+            }
+        }
+        return res;
+    }
+
+    // solve_batch (:272-302)
+    std::vector<BatchItem> solve_batch(const std::vector<PricingParams>& ps, bool shared,
+                                       const std::optional<PDEGridConfig>& custom = std::nullopt) {
+        pde_solves = 0;
+        if (use_normalized && is_normalized_eligible(ps, shared)) return solve_normalized_chain(ps, custom);
+        return solve_regular(ps, shared, custom);
+    }
+};
+
+// European closed form (src/option/european_option.hpp:24-66; european_option.cpp:25-53)
+inline double norm_cdf(double x) { return 0.5 * std::erfc(-x * M_SQRT1_2); }
+inline double norm_pdf(double x) { return 0.3989422804014327 * std::exp(-0.5 * x * x); }
+inline double bs_price(double S, double K, double tau, double sigma, double r, double q, OptionType type) {
+    const double sst = sigma * std::sqrt(tau);
+    const double d1 = (std::log(S / K) + (r - q + 0.5 * sigma * sigma) * tau) / sst;
+    const double d2 = d1 - sigma * std::sqrt(tau);
+    const double eq = std::exp(-q * tau), er = std::exp(-r * tau);
+    if (type == OptionType::PUT) return K * er * norm_cdf(-d2) - S * eq * norm_cdf(-d1);
+    return S * eq * norm_cdf(d1) - K * er * norm_cdf(d2);
+}
+inline double bs_vega(double S, double K, double tau, double sigma, double r, double q) {
+    const double sst = sigma * std::sqrt(tau);
+    const double d1 = (std::log(S / K) + (r - q + 0.5 * sigma * sigma) * tau) / sst;
+    return S * std::exp(-q * tau) * std::sqrt(tau) * norm_pdf(d1);
+}
+}  // namespace oracle

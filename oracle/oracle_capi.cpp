@@ -1,0 +1,269 @@
+// SPDX-License-Identifier: MIT
+//
+// ORACLE — test infrastructure only. extern "C" wrappers over the CPU
+// restatement in mango_oracle.hpp so that pytest (ctypes) and bench.py's CPU
+// baseline can call it. Uses the struct layouts of include/mango_amd.h.
+#include "mango_oracle.hpp"
+
+#include "../include/mango_amd.h"
+
+#include <chrono>
+#include <cstring>
+
+using namespace oracle;
+
+namespace {
+
+PricingParams to_params(const MangoPricingParams& p) {
+    PricingParams o;
+    o.spot = p.spot;
+    o.strike = p.strike;
+    o.maturity = p.maturity;
+    o.rate = p.rate_const;
+    o.dividend_yield = p.dividend_yield;
+    o.volatility = p.volatility;
+    o.type = p.option_type == MANGO_PUT ? OptionType::PUT : OptionType::CALL;
+    return o;
+}
+
+GridAccuracyParams to_acc(const MangoGridAccuracy* a) {
+    GridAccuracyParams g;
+    if (!a) return g;
+    g.n_sigma = a->n_sigma;
+    g.alpha = a->alpha;
+    g.tol = a->tol;
+    g.c_t = a->c_t;
+    g.min_spatial_points = a->min_spatial_points;
+    g.max_spatial_points = a->max_spatial_points;
+    g.max_time_steps = a->max_time_steps;
+    return g;
+}
+
+GridSpec to_spec(const MangoGridSpec& s) {
+    GridSpec g;
+    g.type = static_cast<GridSpec::Type>(s.type);
+    g.x_min = s.x_min;
+    g.x_max = s.x_max;
+    g.n_points = s.n_points;
+    g.concentration = s.concentration;
+    for (int i = 0; i < s.n_clusters; ++i)
+        g.clusters.push_back({s.clusters[i].center_x, s.clusters[i].alpha, s.clusters[i].weight});
+    return g;
+}
+
+void from_spec(const GridSpec& g, MangoGridSpec* s) {
+    std::memset(s, 0, sizeof(*s));
+    s->type = static_cast<int32_t>(g.type);
+    s->x_min = g.x_min;
+    s->x_max = g.x_max;
+    s->n_points = g.n_points;
+    s->concentration = g.concentration;
+    s->n_clusters = static_cast<int32_t>(g.clusters.size());
+    for (size_t i = 0; i < g.clusters.size() && i < MANGO_MAX_CLUSTERS; ++i) {
+        s->clusters[i].center_x = g.clusters[i].center_x;
+        s->clusters[i].alpha = g.clusters[i].alpha;
+        s->clusters[i].weight = g.clusters[i].weight;
+    }
+}
+
+int32_t map_status(int32_t s) {
+    switch (s) {
+        case kOk: return MANGO_OK;
+        case kValidation: return MANGO_ERR_VALIDATION;
+        default: return MANGO_ERR_SOLVER;
+    }
+}
+
+constexpr double kNaN = std::numeric_limits<double>::quiet_NaN();
+
+}  // namespace
+
+extern "C" {
+
+int32_t oracle_generate(const MangoGridSpec* spec, double* out_x) {
+    const auto g = to_spec(*spec);
+    const auto x = g.generate();
+    std::copy(x.begin(), x.end(), out_x);
+    return 0;
+}
+
+int32_t oracle_multi_sinh(double x_min, double x_max, uint64_t n, const MangoSinhCluster* cl,
+                          uint64_t nc, MangoGridSpec* out) {
+    std::vector<Cluster> v;
+    for (uint64_t i = 0; i < nc; ++i) v.push_back({cl[i].center_x, cl[i].alpha, cl[i].weight});
+    auto g = GridSpec::multi_sinh_spaced(x_min, x_max, n, v);
+    if (!g) return 1;
+    from_spec(*g, out);
+    return 0;
+}
+
+int32_t oracle_estimate_pde_grid(const MangoPricingParams* p, const MangoGridAccuracy* acc,
+                                 MangoGridSpec* out, uint64_t* n_time) {
+    auto [g, td] = estimate_pde_grid(to_params(*p), to_acc(acc));
+    from_spec(g, out);
+    *n_time = td.n_steps;
+    return 0;
+}
+
+int32_t oracle_estimate_batch_pde_grid(const MangoPricingParams* p, uint64_t n,
+                                       const MangoGridAccuracy* acc, MangoGridSpec* out,
+                                       uint64_t* n_time) {
+    std::vector<PricingParams> v;
+    for (uint64_t i = 0; i < n; ++i) v.push_back(to_params(p[i]));
+    auto [g, td] = estimate_batch_pde_grid(v, to_acc(acc));
+    from_spec(g, out);
+    *n_time = td.n_steps;
+    return 0;
+}
+
+// One solve on an explicit grid; x/u/u_prev may be null.
+int32_t oracle_solve(const MangoPricingParams* p, const MangoGridSpec* spec, uint64_t n_time,
+                     double* x, double* u, double* u_prev, MangoAmericanGreeks* greeks) {
+    const auto pp = to_params(*p);
+    Solution s;
+    const int32_t st = solve_american(pp, to_spec(*spec), TimeDomain::from_n_steps(0.0, pp.maturity, n_time), s);
+    if (st != kOk) return map_status(st);
+    if (x) std::copy(s.x.begin(), s.x.end(), x);
+    if (u) std::copy(s.u.begin(), s.u.end(), u);
+    if (u_prev) std::copy(s.u_prev.begin(), s.u_prev.end(), u_prev);
+    if (greeks) {
+        const auto g = compute_greeks(s, pp.spot, pp.strike);
+        *greeks = {g.value, g.delta, g.gamma, g.theta};
+    }
+    return MANGO_OK;
+}
+
+// Solve with snapshot recording: out_snap has n_snap * n_points slots and
+// receives the rows of the distinct snapped steps in ascending order;
+// out_snap_times receives their times; *out_n_snap their count.
+int32_t oracle_solve_snapshots(const MangoPricingParams* p, const MangoGridSpec* spec, uint64_t n_time,
+                               const double* times, uint64_t n_times, double* out_snap,
+                               double* out_snap_times, uint64_t* out_n_snap) {
+    const auto pp = to_params(*p);
+    SolveOptions o;
+    o.snapshot_times.assign(times, times + n_times);
+    Solution s;
+    const int32_t st = solve_american(pp, to_spec(*spec), TimeDomain::from_n_steps(0.0, pp.maturity, n_time), s, o);
+    if (st != kOk) return map_status(st);
+    std::copy(s.snapshots.begin(), s.snapshots.end(), out_snap);
+    std::copy(s.snapshot_times.begin(), s.snapshot_times.end(), out_snap_times);
+    *out_n_snap = s.snapshot_indices.size();
+    return MANGO_OK;
+}
+
+// BatchAmericanOptionSolver::solve_batch restated, with the grid modes of
+// MangoGridConfig. Returns the failed count; *pde_solves receives the number
+// of PDE solves actually run.
+uint64_t oracle_price_batch(const MangoPricingParams* p, uint64_t n, int32_t use_shared_grid,
+                            int32_t use_normalized, const MangoGridConfig* grid,
+                            MangoAmericanGreeks* out, int32_t* status, int32_t n_threads,
+                            uint64_t* pde_solves) {
+    std::vector<PricingParams> ps;
+    ps.reserve(n);
+    for (uint64_t i = 0; i < n; ++i) ps.push_back(to_params(p[i]));
+    BatchSolver bs;
+    bs.use_normalized = use_normalized != 0;
+    bs.n_threads = n_threads > 0 ? n_threads : 1;
+    std::vector<BatchItem> res;
+    if (grid && grid->mode == MANGO_GRID_MODE_ACCURACY_FIXED_TIME) {
+        // per-option PDEGridConfig{estimate_pde_grid(p, acc).spec, n_time}
+        const auto acc = to_acc(&grid->accuracy);
+        res.resize(n);
+        #pragma omp parallel for schedule(static) num_threads(bs.n_threads)
+        for (uint64_t i = 0; i < n; ++i) {
+            if (!validate_pricing_params(ps[i])) { res[i].status = kValidation; continue; }
+            auto g = estimate_pde_grid(ps[i], acc);
+            res[i].status = solve_american(ps[i], g.first, TimeDomain::from_n_steps(0.0, ps[i].maturity, grid->n_time), res[i].sol);
+            if (res[i].status == kOk) res[i].greeks = compute_greeks(res[i].sol, ps[i].spot, ps[i].strike);
+        }
+        bs.pde_solves = n;
+    } else if (grid && grid->mode == MANGO_GRID_MODE_ACCURACY && grid->min_time_steps > 0) {
+        // per-option auto grid with a floor on the step count
+        const auto acc = to_acc(&grid->accuracy);
+        res.resize(n);
+        #pragma omp parallel for schedule(dynamic, 16) num_threads(bs.n_threads)
+        for (uint64_t i = 0; i < n; ++i) {
+            if (!validate_pricing_params(ps[i])) { res[i].status = kValidation; continue; }
+            auto g = estimate_pde_grid(ps[i], acc);
+            const size_t nt = std::max<size_t>(g.second.n_steps, grid->min_time_steps);
+            res[i].status = solve_american(ps[i], g.first, TimeDomain::from_n_steps(0.0, ps[i].maturity, nt), res[i].sol);
+            if (res[i].status == kOk) res[i].greeks = compute_greeks(res[i].sol, ps[i].spot, ps[i].strike);
+        }
+        bs.pde_solves = n;
+    } else {
+        std::optional<PDEGridConfig> custom;
+        if (grid && grid->mode == MANGO_GRID_MODE_ACCURACY) bs.accuracy = to_acc(&grid->accuracy);
+        if (grid && grid->mode == MANGO_GRID_MODE_EXPLICIT) custom = PDEGridConfig{to_spec(grid->spec), grid->n_time};
+        res = bs.solve_batch(ps, use_shared_grid != 0, custom);
+    }
+    uint64_t failed = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        status[i] = map_status(res[i].status);
+        if (res[i].status == kOk) {
+            out[i] = {res[i].greeks.value, res[i].greeks.delta, res[i].greeks.gamma, res[i].greeks.theta};
+        } else {
+            out[i] = {kNaN, kNaN, kNaN, kNaN};
+            ++failed;
+        }
+    }
+    if (pde_solves) *pde_solves = bs.pde_solves;
+    return failed;
+}
+
+int32_t oracle_thomas(const double* lo, const double* dg, const double* up, const double* rhs,
+                      double* sol, uint64_t n) {
+    std::vector<double> ws(2 * n + 2);
+    return solve_thomas(lo, dg, up, rhs, sol, n, ws.data()) ? 0 : 1;
+}
+
+int32_t oracle_thomas_projected(const double* lo, const double* dg, const double* up,
+                                const double* rhs, const double* psi, double* sol, uint64_t n) {
+    std::vector<double> ws(2 * n + 2);
+    return solve_thomas_projected(lo, dg, up, rhs, psi, sol, n, ws.data()) ? 0 : 1;
+}
+
+// Natural cubic spline: evaluate value and derivative at m points.
+int32_t oracle_spline_eval(const double* x, const double* y, uint64_t n, const double* xe,
+                           uint64_t m, double* val, double* der) {
+    CubicSpline s;
+    if (!s.build(x, y, n)) return 1;
+    for (uint64_t i = 0; i < m; ++i) {
+        if (val) val[i] = s.eval(xe[i]);
+        if (der) der[i] = s.eval_derivative(xe[i]);
+    }
+    return 0;
+}
+
+// L(u) by the centered stencil and by the analytic Jacobian bands, for the
+// Jacobian-consistency pin (tests/internal/spatial_operator_jacobian_test.cc).
+int32_t oracle_operator(const MangoPricingParams* p, const double* x, uint64_t n, const double* u,
+                        double* lu_stencil, double* jl, double* jd, double* ju) {
+    const auto pp = to_params(*p);
+    std::vector<double> xs(x, x + n);
+    const Spacing sp(xs);
+    std::vector<double> d2(n, 0.0), d1(n, 0.0);
+    sp.second_derivative(u, d2.data(), 1, n - 1);
+    sp.first_derivative(u, d1.data(), 1, n - 1);
+    const double hss = 0.5 * pp.volatility * pp.volatility;
+    const double r = pp.rate, q = pp.dividend_yield;
+    for (uint64_t i = 0; i < n; ++i) { lu_stencil[i] = 0.0; jl[i] = jd[i] = ju[i] = 0.0; }
+    for (uint64_t i = 1; i + 1 < n; ++i) {
+        const double drift = r - q - hss;
+        lu_stencil[i] = hss * d2[i] + drift * d1[i] - r * u[i];
+        const double a = hss, b = r - q - hss, c = -r;
+        const double dl = x[i] - x[i - 1], dr = x[i + 1] - x[i], da = (dl + dr) / 2.0;
+        jl[i] = a / (dl * da) + (-b * dr / (dl * (dl + dr)));
+        jd[i] = -a * (1.0 / dl + 1.0 / dr) / da + b * (dr - dl) / (dl * dr) + c;
+        ju[i] = a / (dr * da) + b * dl / (dr * (dl + dr));
+    }
+    return 0;
+}
+
+double oracle_bs_price(double S, double K, double tau, double sigma, double r, double q, int32_t type) {
+    return bs_price(S, K, tau, sigma, r, q, type == MANGO_PUT ? OptionType::PUT : OptionType::CALL);
+}
+double oracle_bs_vega(double S, double K, double tau, double sigma, double r, double q) {
+    return bs_vega(S, K, tau, sigma, r, q);
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+# SPDX-License-Identifier: MIT
+"""ORACLE — test infrastructure only.
+
+ctypes bindings of liboracle.so, the CPU restatement of the reference's
+American PDE path (mango_oracle.hpp). Only tests/, __graft_entry__.smoke()
+and bench.py's CPU-baseline leg import this module; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from mango_option._abi import (MangoAmericanGreeks, MangoGridAccuracy, MangoGridConfig, MangoGridSpec,
+                               MangoPricingParams, MangoSinhCluster)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    sig = {
+        "oracle_generate": (C.c_int32, [P(MangoGridSpec), _dp]),
+        "oracle_multi_sinh": (C.c_int32, [C.c_double, C.c_double, C.c_uint64, P(MangoSinhCluster), C.c_uint64,
+                                          P(MangoGridSpec)]),
+        "oracle_estimate_pde_grid": (C.c_int32, [P(MangoPricingParams), P(MangoGridAccuracy), P(MangoGridSpec),
+                                                 P(C.c_uint64)]),
+        "oracle_estimate_batch_pde_grid": (C.c_int32, [P(MangoPricingParams), C.c_uint64, P(MangoGridAccuracy),
+                                                       P(MangoGridSpec), P(C.c_uint64)]),
+        "oracle_solve": (C.c_int32, [P(MangoPricingParams), P(MangoGridSpec), C.c_uint64, _dp, _dp, _dp,
+                                     P(MangoAmericanGreeks)]),
+        "oracle_solve_snapshots": (C.c_int32, [P(MangoPricingParams), P(MangoGridSpec), C.c_uint64, _dp,
+                                               C.c_uint64, _dp, _dp, P(C.c_uint64)]),
+        "oracle_price_batch": (C.c_uint64, [P(MangoPricingParams), C.c_uint64, C.c_int32, C.c_int32,
+                                            P(MangoGridConfig), P(MangoAmericanGreeks), P(C.c_int32), C.c_int32,
+                                            P(C.c_uint64)]),
+        "oracle_thomas": (C.c_int32, [_dp, _dp, _dp, _dp, _dp, C.c_uint64]),
+        "oracle_thomas_projected": (C.c_int32, [_dp, _dp, _dp, _dp, _dp, _dp, C.c_uint64]),
+        "oracle_spline_eval": (C.c_int32, [_dp, _dp, C.c_uint64, _dp, C.c_uint64, _dp, _dp]),
+        "oracle_operator": (C.c_int32, [P(MangoPricingParams), _dp, C.c_uint64, _dp, _dp, _dp, _dp, _dp]),
+        "oracle_bs_price": (C.c_double, [C.c_double] * 6 + [C.c_int32]),
+        "oracle_bs_vega": (C.c_double, [C.c_double] * 6),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _a(x) -> np.ndarray:
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def _p(x: np.ndarray):
+    return x.ctypes.data_as(_dp)
+
+
+def thomas(lo, dg, up, rhs):
+    lo, dg, up, rhs = _a(lo), _a(dg), _a(up), _a(rhs)
+    sol = np.zeros(len(dg))
+    ok = load().oracle_thomas(_p(lo), _p(dg), _p(up), _p(rhs), _p(sol), len(dg)) == 0
+    return ok, sol
+
+
+def thomas_projected(lo, dg, up, rhs, psi):
+    lo, dg, up, rhs, psi = _a(lo), _a(dg), _a(up), _a(rhs), _a(psi)
+    sol = np.zeros(len(dg))
+    ok = load().oracle_thomas_projected(_p(lo), _p(dg), _p(up), _p(rhs), _p(psi), _p(sol), len(dg)) == 0
+    return ok, sol
+
+
+def spline_eval(x, y, xe):
+    x, y, xe = _a(x), _a(y), _a(xe)
+    v = np.zeros(len(xe)); d = np.zeros(len(xe))
+    ok = load().oracle_spline_eval(_p(x), _p(y), len(x), _p(xe), len(xe), _p(v), _p(d)) == 0
+    return ok, v, d
+
+
+def generate(spec: MangoGridSpec) -> np.ndarray:
+    x = np.zeros(spec.n_points)
+    load().oracle_generate(C.byref(spec), _p(x))
+    return x
+
+
+def multi_sinh(x_min, x_max, n, clusters) -> MangoGridSpec | None:
+    arr = (MangoSinhCluster * max(1, len(clusters)))(*[MangoSinhCluster(*c) for c in clusters])
+    out = MangoGridSpec()
+    return out if load().oracle_multi_sinh(x_min, x_max, n, arr, len(clusters), C.byref(out)) == 0 else None
+
+
+def c_params(options) -> tuple:
+    """Array of MangoPricingParams from an OptionBatch."""
+    n = len(options)
+    arr = (MangoPricingParams * max(1, n))()
+    for i in range(n):
+        p = arr[i]
+        p.spot = float(options.spot[i]); p.strike = float(options.strike[i])
+        p.maturity = float(options.maturity[i]); p.dividend_yield = float(options.dividend_yield[i])
+        p.volatility = float(options.volatility[i]); p.rate_const = float(options.rate[i])
+        p.option_type = int(options.option_type[i])
+    return arr, n
+
+
+def one_param(spot, strike, maturity, vol, rate, q, put=True) -> MangoPricingParams:
+    p = MangoPricingParams()
+    p.spot, p.strike, p.maturity, p.volatility, p.rate_const, p.dividend_yield = spot, strike, maturity, vol, rate, q
+    p.option_type = 1 if put else 0
+    return p
+
+
+def estimate_pde_grid(p: MangoPricingParams, acc: MangoGridAccuracy | None = None):
+    spec = MangoGridSpec()
+    nt = C.c_uint64()
+    load().oracle_estimate_pde_grid(C.byref(p), C.byref(acc) if acc is not None else None, C.byref(spec),
+                                    C.byref(nt))
+    return spec, nt.value
+
+
+def estimate_batch_pde_grid(arr, n, acc: MangoGridAccuracy | None = None):
+    spec = MangoGridSpec()
+    nt = C.c_uint64()
+    load().oracle_estimate_batch_pde_grid(arr, n, C.byref(acc) if acc is not None else None, C.byref(spec),
+                                          C.byref(nt))
+    return spec, nt.value
+
+
+def solve(p: MangoPricingParams, spec: MangoGridSpec, n_time: int):
+    n = spec.n_points
+    x = np.zeros(n); u = np.zeros(n); up = np.zeros(n)
+    g = MangoAmericanGreeks()
+    st = load().oracle_solve(C.byref(p), C.byref(spec), n_time, _p(x), _p(u), _p(up), C.byref(g))
+    return st, {"x": x, "u": u, "u_prev": up, "greeks": np.array([g.value, g.delta, g.gamma, g.theta])}
+
+
+def solve_snapshots(p: MangoPricingParams, spec: MangoGridSpec, n_time: int, times):
+    times = _a(times)
+    n = spec.n_points
+    snap = np.zeros((len(times), n)); st_t = np.zeros(len(times)); cnt = C.c_uint64()
+    st = load().oracle_solve_snapshots(C.byref(p), C.byref(spec), n_time, _p(times), len(times), _p(snap),
+                                       _p(st_t), C.byref(cnt))
+    k = cnt.value
+    return st, snap[:k], st_t[:k]
+
+
+def price_batch(options, use_shared_grid=False, grid: MangoGridConfig | None = None, use_normalized=True,
+                n_threads=1, params_array=None):
+    """Returns (greeks[n,4], status[n], failed, pde_solves)."""
+    if params_array is None:
+        arr, n = c_params(options)
+    else:
+        arr, n = params_array
+    out = (MangoAmericanGreeks * max(1, n))()
+    status = (C.c_int32 * max(1, n))()
+    solves = C.c_uint64()
+    failed = load().oracle_price_batch(arr, n, int(use_shared_grid), int(use_normalized),
+                                       C.byref(grid) if grid is not None else None, out, status, n_threads,
+                                       C.byref(solves))
+    g = np.frombuffer(out, dtype=np.float64, count=4 * n).reshape(n, 4).copy()
+    s = np.frombuffer(status, dtype=np.int32, count=n).copy()
+    return g, s, int(failed), int(solves.value)
+
+
+def operator(p: MangoPricingParams, x, u):
+    x, u = _a(x), _a(u)
+    n = len(x)
+    lu = np.zeros(n); jl = np.zeros(n); jd = np.zeros(n); ju = np.zeros(n)
+    load().oracle_operator(C.byref(p), _p(x), n, _p(u), _p(lu), _p(jl), _p(jd), _p(ju))
+    return lu, jl, jd, ju
+
+
+def bs_price(S, K, tau, sigma, r, q, put=True) -> float:
+    return load().oracle_bs_price(S, K, tau, sigma, r, q, 1 if put else 0)
+
+
+def bs_vega(S, K, tau, sigma, r, q) -> float:
+    return load().oracle_bs_vega(S, K, tau, sigma, r, q)
