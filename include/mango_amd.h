@@ -222,6 +222,27 @@ MangoStatus mango_price_american_batch_device(const MangoDeviceOption* d_opts, u
                                               MangoAmericanGreeks* d_out, int32_t* d_status,
                                               void* stream, MangoError* err);
 
+/* Shared-grid chains with inputs in HBM: segment s is options
+ * [d_seg_offsets[s], d_seg_offsets[s+1]) and is priced like
+ * solve_batch(segment, use_shared_grid = true, nullptr, grid)
+ * (src/option/american_option_batch.cpp:272-382): one grid per segment (the
+ * EXPLICIT spec, or the spatial union of estimate_batch_pde_grid for
+ * ACCURACY_FIXED_TIME, src/option/grid_spec_types.cpp:120-177), one PDE per
+ * distinct (sigma, r, q, T, type) inside the segment, then each option's
+ * Greeks from its group's solution at ln(S/K). d_seg_offsets has
+ * n_segments + 1 entries (device). Asynchronous on `stream`. Calls share the
+ * library's device scratch: do not overlap calls from several threads. */
+MangoStatus mango_price_american_chains_device(const MangoDeviceOption* d_opts, uint64_t n,
+                                               const int64_t* d_seg_offsets, uint64_t n_segments,
+                                               const MangoGridConfig* grid, MangoAmericanGreeks* d_out,
+                                               int32_t* d_status, void* stream, MangoError* err);
+
+/* Benchmark instrumentation: when enabled, the device entry points bracket
+ * every launch of the PDE solve kernel with HIP events on its stream;
+ * mango_profile_read synchronizes them and returns the summed kernel time. */
+void mango_profile_enable(int32_t enable);
+MangoStatus mango_profile_read(double* solve_kernel_ms, uint64_t* solve_launches);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

@@ -264,6 +264,7 @@ struct StageCache {
 
 template <int K, int M>
 __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __restrict__ jobs, int n_jobs,
+                                                                const int32_t* __restrict__ d_n_jobs,
                                                                 Greeks* __restrict__ greeks_out,
                                                                 int32_t* __restrict__ status_out,
                                                                 StateOut state) {
@@ -277,6 +278,8 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
     const int tid = threadIdx.x;
     const int slot = tid / K;
     const int lane = tid % K;
+    if (d_n_jobs) n_jobs = min(n_jobs, *d_n_jobs);
+    if (static_cast<int>(blockIdx.x) * GPB >= n_jobs) return;  // block-uniform: no barrier is skipped
     int job_idx = blockIdx.x * GPB + slot;
     const bool live = job_idx < n_jobs;
     if (!live) job_idx = n_jobs - 1;
@@ -716,6 +719,99 @@ __global__ __launch_bounds__(kBlock) void chain_epilogue_kernel(const ChainOptio
     status_out[o.out_index] = kStatusOk;
 }
 
+// Chain planner: one thread per segment of options that shares one grid
+// (BatchAmericanOptionSolver::solve_batch(segment, use_shared_grid = true)).
+// The shared grid is the spatial part of estimate_batch_pde_grid
+// (grid_spec_types.cpp:120-177); options are grouped by (sigma, r, q, T,
+// type) within 1e-10 like group_by_pde_parameters (american_option_batch.cpp:
+// 219-270) and one S = K = 1 PDE is emitted per group. On one shared grid the
+// reference's regular route gives each member the same u(x) (the PDE is
+// strike-free), so deduplicating is exact for either route.
+__global__ __launch_bounds__(kBlock) void plan_chains_kernel(const DeviceOptionRecord* __restrict__ opts,
+                                                             const int64_t* __restrict__ seg_offsets, int n_segments,
+                                                             DevicePlan plan, PdeJob* __restrict__ jobs,
+                                                             int32_t* __restrict__ job_count,
+                                                             ChainOption* __restrict__ copts,
+                                                             int32_t* __restrict__ rep) {
+    const int sg = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sg >= n_segments) return;
+    const int64_t b = seg_offsets[sg], e = seg_offsets[sg + 1];
+    if (e <= b) return;
+    double gx_min = 1.7976931348623157e308, gx_max = -1.7976931348623157e308;
+    int64_t gnx = 0;
+    auto valid = [](const DeviceOptionRecord& o) {
+        return o.spot > 0.0 && o.strike > 0.0 && o.maturity > 0.0 && o.volatility > 0.0 && isfinite(o.spot) &&
+               isfinite(o.strike) && isfinite(o.maturity) && isfinite(o.volatility);
+    };
+    for (int64_t i = b; i < e; ++i) {
+        const DeviceOptionRecord o = opts[i];
+        if (!valid(o)) continue;
+        const GridDesc g = estimate_spatial(o.spot, o.strike, o.maturity, o.volatility, plan.accuracy);
+        gx_min = smin(gx_min, g.x_min);
+        gx_max = smax(gx_max, g.x_max);
+        gnx = gnx > g.n_points ? gnx : g.n_points;
+    }
+    GridDesc shared;
+    if (plan.mode == 1) {
+        shared = plan.spec;
+    } else {
+        const double c[1] = {0.0}, a[1] = {plan.accuracy.alpha}, w[1] = {1.0};
+        if (!make_multi_sinh(shared, gx_min, gx_max, gnx, c, a, w, 1)) {
+            shared = GridDesc{};
+            shared.type = kSinh; shared.x_min = gx_min; shared.x_max = gx_max; shared.n_points = gnx;
+            shared.concentration = plan.accuracy.alpha; shared.n_clusters = 0;
+        }
+    }
+    // greedy first-match grouping; rep[i] = index of the group's first member
+    constexpr double TOL = 1e-10;
+    int32_t n_groups = 0;
+    for (int64_t i = b; i < e; ++i) {
+        const DeviceOptionRecord o = opts[i];
+        int32_t r = static_cast<int32_t>(i);
+        for (int64_t j = b; j < i; ++j) {
+            if (rep[j] != j) continue;
+            const DeviceOptionRecord p = opts[j];
+            if (fabs(p.volatility - o.volatility) < TOL && fabs(p.rate - o.rate) < TOL &&
+                fabs(p.dividend_yield - o.dividend_yield) < TOL && fabs(p.maturity - o.maturity) < TOL &&
+                p.option_type == o.option_type) {
+                r = static_cast<int32_t>(j);
+                break;
+            }
+        }
+        rep[i] = r;
+        if (r == i) ++n_groups;
+    }
+    const int32_t base = atomicAdd(job_count, n_groups);
+    int32_t ord = 0;
+    for (int64_t i = b; i < e; ++i) {
+        if (rep[i] != i) continue;
+        const DeviceOptionRecord o = opts[i];
+        PdeJob j;
+        j.maturity = o.maturity; j.volatility = o.volatility; j.rate = o.rate; j.dividend_yield = o.dividend_yield;
+        j.spot = 1.0; j.strike = 1.0; j.c_t = plan.accuracy.c_t; j.pad0 = 0.0;
+        j.grid = shared;
+        j.option_type = o.option_type;
+        j.n_time = plan.n_time;
+        j.max_time_steps = plan.accuracy.max_time_steps;
+        j.min_time_steps = 0;
+        j.out_index = base + ord;
+        j.flags = kJobWriteState;
+        jobs[base + ord] = j;
+        rep[i] = -(base + ord) - 1;  // mark: group slot of this representative
+        ++ord;
+    }
+    for (int64_t i = b; i < e; ++i) {
+        const int32_t r = rep[i];
+        const int32_t slot = r < 0 ? -r - 1 : -rep[r] - 1;
+        ChainOption c;
+        c.spot = opts[i].spot;
+        c.strike = opts[i].strike;
+        c.group = slot;
+        c.out_index = static_cast<int32_t>(i);
+        copts[i] = c;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void plan_jobs_kernel(const DeviceOptionRecord* __restrict__ opts, int n,
                                                            DevicePlan plan, PdeJob* __restrict__ jobs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -753,12 +849,12 @@ __global__ __launch_bounds__(kBlock) void plan_jobs_kernel(const DeviceOptionRec
 }
 
 template <int K, int M>
-int launch_variant(const PdeJob* d_jobs, int n_jobs, Greeks* d_greeks, int32_t* d_status,
+int launch_variant(const PdeJob* d_jobs, int n_jobs, const int32_t* d_n_jobs, Greeks* d_greeks, int32_t* d_status,
                    const StateOut& state, hipStream_t stream) {
     constexpr int GPB = kBlock / K;
     const int blocks = (n_jobs + GPB - 1) / GPB;
     hipLaunchKernelGGL((american_solve_kernel<K, M>), dim3(blocks), dim3(kBlock), 0, stream, d_jobs, n_jobs,
-                       d_greeks, d_status, state);
+                       d_n_jobs, d_greeks, d_status, state);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -778,13 +874,13 @@ SolveVariant pick_variant(int64_t n_space) {
 }
 
 int launch_american_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, Greeks* d_greeks,
-                          int32_t* d_status, const StateOut& state, hipStream_t stream) {
+                          int32_t* d_status, const StateOut& state, hipStream_t stream, const int32_t* d_n_jobs) {
     if (n_jobs <= 0) return 0;
-    if (v.lanes == 64 && v.nodes == 2) return launch_variant<64, 2>(d_jobs, n_jobs, d_greeks, d_status, state, stream);
-    if (v.lanes == 64 && v.nodes == 4) return launch_variant<64, 4>(d_jobs, n_jobs, d_greeks, d_status, state, stream);
-    if (v.lanes == 64 && v.nodes == 7) return launch_variant<64, 7>(d_jobs, n_jobs, d_greeks, d_status, state, stream);
-    if (v.lanes == 32 && v.nodes == 4) return launch_variant<32, 4>(d_jobs, n_jobs, d_greeks, d_status, state, stream);
-    if (v.lanes == 16 && v.nodes == 8) return launch_variant<16, 8>(d_jobs, n_jobs, d_greeks, d_status, state, stream);
+    if (v.lanes == 64 && v.nodes == 2) return launch_variant<64, 2>(d_jobs, n_jobs, d_n_jobs, d_greeks, d_status, state, stream);
+    if (v.lanes == 64 && v.nodes == 4) return launch_variant<64, 4>(d_jobs, n_jobs, d_n_jobs, d_greeks, d_status, state, stream);
+    if (v.lanes == 64 && v.nodes == 7) return launch_variant<64, 7>(d_jobs, n_jobs, d_n_jobs, d_greeks, d_status, state, stream);
+    if (v.lanes == 32 && v.nodes == 4) return launch_variant<32, 4>(d_jobs, n_jobs, d_n_jobs, d_greeks, d_status, state, stream);
+    if (v.lanes == 16 && v.nodes == 8) return launch_variant<16, 8>(d_jobs, n_jobs, d_n_jobs, d_greeks, d_status, state, stream);
     return 2;
 }
 
@@ -793,6 +889,15 @@ int launch_plan_jobs(const DeviceOptionRecord* d_opts, int n, const DevicePlan& 
     if (n <= 0) return 0;
     hipLaunchKernelGGL(plan_jobs_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d_opts, n, plan,
                        d_jobs);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_plan_chains(const DeviceOptionRecord* d_opts, const int64_t* d_seg_offsets, int n_segments,
+                       const DevicePlan& plan, PdeJob* d_jobs, int32_t* d_job_count, ChainOption* d_copts,
+                       int32_t* d_rep, hipStream_t stream) {
+    if (n_segments <= 0) return 0;
+    hipLaunchKernelGGL(plan_chains_kernel, dim3((n_segments + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d_opts,
+                       d_seg_offsets, n_segments, plan, d_jobs, d_job_count, d_copts, d_rep);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

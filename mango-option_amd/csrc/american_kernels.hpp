@@ -64,9 +64,10 @@ struct SolveVariant {
 };
 
 // Host launchers (american_kernels.hip).
+// d_n_jobs (optional, device): the kernel processes min(n_jobs, *d_n_jobs) jobs.
 int launch_american_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs,
                           Greeks* d_greeks, int32_t* d_status, const StateOut& state,
-                          hipStream_t stream);
+                          hipStream_t stream, const int32_t* d_n_jobs = nullptr);
 bool variant_supported(const SolveVariant& v);
 int max_nodes_supported();
 SolveVariant pick_variant(int64_t n_space);
@@ -104,5 +105,11 @@ struct DeviceOptionRecord {  // layout of MangoDeviceOption
 
 int launch_plan_jobs(const DeviceOptionRecord* d_opts, int n, const DevicePlan& plan, PdeJob* d_jobs,
                      hipStream_t stream);
+// Segments [seg_offsets[s], seg_offsets[s+1]) each share one grid; emits one
+// job per distinct PDE (count in *d_job_count, which must start at 0) and one
+// ChainOption per option. d_rep is scratch of n ints.
+int launch_plan_chains(const DeviceOptionRecord* d_opts, const int64_t* d_seg_offsets, int n_segments,
+                       const DevicePlan& plan, PdeJob* d_jobs, int32_t* d_job_count, ChainOption* d_copts,
+                       int32_t* d_rep, hipStream_t stream);
 
 }  // namespace mango_amd

@@ -177,6 +177,9 @@ struct DevBuf {
 };
 }  // namespace
 
+int timed_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, Greeks* d_greeks, int32_t* d_status,
+                const StateOut& state, hipStream_t stream, const int32_t* d_n_jobs);
+
 bool device_ready(MangoError* err) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_device_ok < 0) {
@@ -434,8 +437,120 @@ MangoStatus device_batch(const MangoDeviceOption* d_opts, uint64_t n, const Mang
     if (!d_jobs) return MANGO_ERR_SOLVER;
     if (launch_plan_jobs(reinterpret_cast<const DeviceOptionRecord*>(d_opts), static_cast<int>(n), plan, d_jobs,
                          stream) != 0 ||
-        launch_american_solve(pick_variant(n_max), d_jobs, static_cast<int>(n), reinterpret_cast<Greeks*>(d_out),
-                              d_status, StateOut{}, stream) != 0) {
+        timed_solve(pick_variant(n_max), d_jobs, static_cast<int>(n), reinterpret_cast<Greeks*>(d_out), d_status,
+                    StateOut{}, stream, nullptr) != 0) {
+        set_err(err, MANGO_ERR_SOLVER, (std::string("HIP launch error: ") + hipGetErrorString(hipGetLastError())).c_str());
+        return MANGO_ERR_SOLVER;
+    }
+    return MANGO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Solve-kernel timing (HIP events on the launch stream), for benchmarks.
+// ---------------------------------------------------------------------------
+namespace {
+bool g_timing = false;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_events;
+}  // namespace
+
+int timed_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, Greeks* d_greeks, int32_t* d_status,
+                const StateOut& state, hipStream_t stream, const int32_t* d_n_jobs) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (g_timing) {
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        (void)hipEventRecord(a, stream);
+    }
+    const int rc = launch_american_solve(v, d_jobs, n_jobs, d_greeks, d_status, state, stream, d_n_jobs);
+    if (g_timing) {
+        (void)hipEventRecord(b, stream);
+        g_events.push_back({a, b});
+    }
+    return rc;
+}
+
+// Grow-only device scratch for the device-resident chain path.
+struct ChainScratch {
+    size_t cap = 0;
+    int64_t stride = 0;
+    PdeJob* jobs = nullptr;
+    int32_t* job_count = nullptr;
+    ChainOption* copts = nullptr;
+    int32_t* rep = nullptr;
+    int32_t* gstatus = nullptr;
+    double* st = nullptr;      // 5 state arrays of cap*stride
+    double* dt = nullptr;
+    int64_t* ns = nullptr;
+    void release() {
+        for (void* p : {static_cast<void*>(jobs), static_cast<void*>(job_count), static_cast<void*>(copts),
+                        static_cast<void*>(rep), static_cast<void*>(gstatus), static_cast<void*>(st),
+                        static_cast<void*>(dt), static_cast<void*>(ns)})
+            if (p) (void)hipFree(p);
+        *this = ChainScratch{};
+    }
+    bool ensure(size_t n, int64_t s) {
+        if (n <= cap && s <= stride) return true;
+        release();
+        bool ok = hipMalloc(reinterpret_cast<void**>(&jobs), n * sizeof(PdeJob)) == hipSuccess;
+        ok = ok && hipMalloc(reinterpret_cast<void**>(&job_count), sizeof(int32_t)) == hipSuccess;
+        ok = ok && hipMalloc(reinterpret_cast<void**>(&copts), n * sizeof(ChainOption)) == hipSuccess;
+        ok = ok && hipMalloc(reinterpret_cast<void**>(&rep), n * sizeof(int32_t)) == hipSuccess;
+        ok = ok && hipMalloc(reinterpret_cast<void**>(&gstatus), n * sizeof(int32_t)) == hipSuccess;
+        ok = ok && hipMalloc(reinterpret_cast<void**>(&st), 5 * n * static_cast<size_t>(s) * sizeof(double)) == hipSuccess;
+        ok = ok && hipMalloc(reinterpret_cast<void**>(&dt), n * sizeof(double)) == hipSuccess;
+        ok = ok && hipMalloc(reinterpret_cast<void**>(&ns), n * sizeof(int64_t)) == hipSuccess;
+        if (!ok) { release(); return false; }
+        cap = n;
+        stride = s;
+        return true;
+    }
+};
+ChainScratch g_chain;
+
+MangoStatus device_chains(const MangoDeviceOption* d_opts, uint64_t n, const int64_t* d_seg_offsets,
+                          uint64_t n_segments, const MangoGridConfig* grid, MangoAmericanGreeks* d_out,
+                          int32_t* d_status, hipStream_t stream, MangoError* err) {
+    if (n == 0 || n_segments == 0) return MANGO_OK;
+    if (!d_opts || !d_seg_offsets || !d_out || !d_status || !grid) {
+        set_err(err, MANGO_ERR_VALIDATION, "null argument");
+        return MANGO_ERR_VALIDATION;
+    }
+    if (grid->mode != MANGO_GRID_MODE_EXPLICIT && grid->mode != MANGO_GRID_MODE_ACCURACY_FIXED_TIME) {
+        set_err(err, MANGO_ERR_VALIDATION, "device chains need an EXPLICIT or ACCURACY_FIXED_TIME grid");
+        return MANGO_ERR_VALIDATION;
+    }
+    if (grid->n_time == 0) { set_err(err, MANGO_ERR_VALIDATION, "n_time must be > 0"); return MANGO_ERR_VALIDATION; }
+    if (n > static_cast<uint64_t>(std::numeric_limits<int32_t>::max())) {
+        set_err(err, MANGO_ERR_VALIDATION, "batch too large");
+        return MANGO_ERR_VALIDATION;
+    }
+    if (!device_ready(err)) return MANGO_ERR_SOLVER;
+    DevicePlan plan{};
+    plan.mode = grid->mode;
+    plan.n_time = static_cast<int32_t>(grid->n_time);
+    plan.accuracy = to_accuracy(&grid->accuracy);
+    plan.spec = to_desc(grid->spec);
+    int64_t n_max = grid->mode == MANGO_GRID_MODE_EXPLICIT ? plan.spec.n_points : plan.accuracy.max_spatial_points;
+    if (grid->mode != MANGO_GRID_MODE_EXPLICIT && n_max % 2 == 0) ++n_max;
+    if (n_max < 3 || n_max > max_nodes_supported()) {
+        set_err(err, MANGO_ERR_SOLVER, "grid size outside what the GPU solver supports");
+        return MANGO_ERR_SOLVER;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_chain.ensure(n, n_max)) { set_err(err, MANGO_ERR_SOLVER, "hipMalloc failed"); return MANGO_ERR_SOLVER; }
+    const size_t plane = g_chain.cap * static_cast<size_t>(g_chain.stride);
+    StateOut so{g_chain.st, g_chain.st + plane, g_chain.st + 2 * plane, g_chain.st + 3 * plane,
+                g_chain.st + 4 * plane, g_chain.ns, g_chain.dt, g_chain.stride};
+    const int ni = static_cast<int>(n);
+    bool ok = hipMemsetAsync(g_chain.job_count, 0, sizeof(int32_t), stream) == hipSuccess;
+    ok = ok && launch_plan_chains(reinterpret_cast<const DeviceOptionRecord*>(d_opts), d_seg_offsets,
+                                  static_cast<int>(n_segments), plan, g_chain.jobs, g_chain.job_count, g_chain.copts,
+                                  g_chain.rep, stream) == 0;
+    ok = ok && timed_solve(pick_variant(n_max), g_chain.jobs, ni, nullptr, g_chain.gstatus, so, stream,
+                           g_chain.job_count) == 0;
+    ok = ok && launch_chain_epilogue(g_chain.copts, ni, so, g_chain.gstatus, reinterpret_cast<Greeks*>(d_out),
+                                     d_status, stream) == 0;
+    if (!ok) {
         set_err(err, MANGO_ERR_SOLVER, (std::string("HIP launch error: ") + hipGetErrorString(hipGetLastError())).c_str());
         return MANGO_ERR_SOLVER;
     }
@@ -816,6 +931,32 @@ MangoStatus mango_price_american_batch_device(const MangoDeviceOption* d_opts, u
                                               MangoAmericanGreeks* d_out, int32_t* d_status, void* stream,
                                               MangoError* err) {
     return device_batch(d_opts, n, grid, d_out, d_status, static_cast<hipStream_t>(stream), err);
+}
+
+MangoStatus mango_price_american_chains_device(const MangoDeviceOption* d_opts, uint64_t n,
+                                               const int64_t* d_seg_offsets, uint64_t n_segments,
+                                               const MangoGridConfig* grid, MangoAmericanGreeks* d_out,
+                                               int32_t* d_status, void* stream, MangoError* err) {
+    return device_chains(d_opts, n, d_seg_offsets, n_segments, grid, d_out, d_status,
+                         static_cast<hipStream_t>(stream), err);
+}
+
+void mango_profile_enable(int32_t enable) {
+    for (auto& [a, b] : g_events) { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    g_events.clear();
+    g_timing = enable != 0;
+}
+
+MangoStatus mango_profile_read(double* solve_kernel_ms, uint64_t* solve_launches) {
+    double total = 0.0;
+    for (auto& [a, b] : g_events) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) return MANGO_ERR_SOLVER;
+        total += ms;
+    }
+    if (solve_kernel_ms) *solve_kernel_ms = total;
+    if (solve_launches) *solve_launches = g_events.size();
+    return MANGO_OK;
 }
 
 }  // extern "C"
