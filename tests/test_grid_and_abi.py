@@ -1,0 +1,112 @@
+# SPDX-License-Identifier: MIT
+"""CPU tests: grid generation (oracle and the library's host planner agree),
+the C ABI library loads and exports every declared symbol, struct layouts,
+workload generator determinism. No compute call touches a GPU."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import mango_option as M
+from mango_option import _abi, workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    src = open(os.path.join(ROOT, "include", "mango_amd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mango_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(_abi.LIB_PATH)
+    names = _declared_functions()
+    assert len(names) >= 19
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_abi.SIGNATURES), set(names) ^ set(_abi.SIGNATURES)
+
+
+def test_struct_layouts_match_reference_abi():
+    # src/ffi/mango_c_api.h:195-214
+    assert C.sizeof(_abi.MangoPricingParams) == 88
+    assert _abi.MangoPricingParams.tenor_points.offset == 48
+    assert _abi.MangoPricingParams.n_dividends.offset == 72
+    assert _abi.MangoPricingParams.option_type.offset == 80
+    assert C.sizeof(_abi.MangoError) == 260 and _abi.MangoError.message.offset == 4
+
+
+def test_uniform_grid_closed_form():
+    x = M.GridSpec.uniform(-1.0, 1.0, 11).generate()
+    np.testing.assert_allclose(x, np.linspace(-1, 1, 11), atol=1e-15)
+    np.testing.assert_array_equal(x, O.generate(M.GridSpec.uniform(-1.0, 1.0, 11).to_c()))
+
+
+def test_sinh_grid_closed_form_symmetry():
+    c = 3.0
+    spec = M.GridSpec.sinh_spaced(-2.0, 2.0, 41, c)
+    x = spec.generate()
+    eta = np.arange(41) / 40.0
+    want = -2.0 + 4.0 * (1 + np.sinh(c * (eta - 0.5)) / np.sinh(c / 2)) / 2
+    np.testing.assert_allclose(x, want, atol=1e-14)
+    np.testing.assert_allclose(x, -x[::-1], atol=1e-14)
+    assert (np.diff(x) > 0).all()
+
+
+@pytest.mark.parametrize("c2,expect", [(0.05, 1), (0.2, 2)])
+def test_cluster_merge_rule(c2, expect):  # src/pde/core/grid.hpp:191-235
+    a = _abi.default_sinh_alpha()
+    spec = M.GridSpec.multi_sinh_spaced(-1.0, 1.0, 101, [(0.0, a, 1.0), (c2, a, 0.5)])
+    ref = O.multi_sinh(-1.0, 1.0, 101, [(0.0, a, 1.0), (c2, a, 0.5)])
+    assert len(spec.clusters) == expect == ref.n_clusters
+    if expect == 1:
+        assert abs(spec.clusters[0][0] - (1.0 * 0.0 + 0.5 * c2) / 1.5) < 1e-15
+        assert spec.clusters[0][2] == 1.5
+    x = spec.generate()
+    np.testing.assert_array_equal(x, O.generate(spec.to_c()))
+    assert (np.diff(x) > 0).all() and x[0] == -1.0 and x[-1] == 1.0
+
+
+def test_host_estimate_matches_oracle():
+    opts = W.c3_uneven(200, seed=9)
+    arr, n = O.c_params(opts)
+    lib = _abi.load_library()
+    for i in range(n):
+        spec_o, nt_o = O.estimate_pde_grid(arr[i])
+        spec = _abi.MangoGridSpec()
+        nt = C.c_uint64()
+        err = _abi.MangoError()
+        assert lib.mango_estimate_pde_grid(C.byref(arr[i]), None, C.byref(spec), C.byref(nt), C.byref(err)) == 0
+        assert nt.value == nt_o and spec.n_points == spec_o.n_points and spec.n_clusters == spec_o.n_clusters
+        assert spec.x_min == spec_o.x_min and spec.x_max == spec_o.x_max
+        np.testing.assert_array_equal(M.GridSpec.from_c(spec).generate(), O.generate(spec_o))
+
+
+def test_workload_generator_is_deterministic():
+    g = W.Xoshiro256ss(0)
+    first = [g.next() for _ in range(3)]
+    g2 = W.Xoshiro256ss(0)
+    assert first == [g2.next() for _ in range(3)]
+    a = W.c1_puts(64, seed=0)
+    b = W.c1_puts(64, seed=0)
+    assert np.array_equal(a.strike, b.strike)
+    assert ((80 <= a.strike) & (a.strike <= 120)).all() and ((0.1 <= a.volatility) & (a.volatility <= 0.5)).all()
+    opts, chain = W.c2_chains()
+    assert len(opts) == 4096 and len(set(chain)) == 64
+
+
+def test_no_gpu_call_reports_error_loudly():
+    # Without a device the library must refuse to compute (no CPU fallback).
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        M.price_batch(W.c1_puts(2), False, None)
