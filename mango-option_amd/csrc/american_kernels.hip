@@ -251,17 +251,109 @@ __device__ bool params_valid(const PdeJob& j) {
 }
 
 // ---------------------------------------------------------------------------
-// The solve kernel
+// Cross-lane building blocks. A group of K lanes spans K/16 DPP rows. Inside
+// a row DPP row_shr/row_shl move values at VALU latency; across rows the
+// prefix direction uses row_bcast and the suffix direction an indexed
+// shuffle of the next row's first lane.
 // ---------------------------------------------------------------------------
-template <int K, int M>
-struct StageCache {
-    double cp[M];
-    double inv[M];
-    uint32_t mask;
-    bool valid;
-    bool singular;
-};
+constexpr int kRowShl = 0x100;
+constexpr int kRowShr = 0x110;
+constexpr int kWaveShl1 = 0x130;
+constexpr int kWaveShr1 = 0x138;
+constexpr int kRowBcast15 = 0x142;
+constexpr int kRowBcast31 = 0x143;
 
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp(double old, double v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xf, false);
+}
+
+// Affine map x -> A + B x (forward-elimination recurrence of d').
+struct Aff {
+    double A, B;
+};
+__device__ __forceinline__ Aff compose(const Aff& f, const Aff& q) { return {fma(f.B, q.A, f.A), f.B * q.B}; }
+
+// Clamped-affine map v -> min(max(a v + b, lo), hi) (projected back substitution).
+struct Clamp {
+    double a, b, lo, hi;
+};
+__device__ __forceinline__ Clamp clamp_identity() { return {1.0, 0.0, -kBig, kBig}; }
+// f o q. For either sign of f.a the image of [q.lo, q.hi] is [min(t1,t2), max(t1,t2)].
+__device__ __forceinline__ Clamp compose(const Clamp& f, const Clamp& q) {
+    const double t1 = fma(f.a, q.lo, f.b);
+    const double t2 = fma(f.a, q.hi, f.b);
+    return {f.a * q.a, fma(f.a, q.b, f.b), fmin(fmax(fmin(t1, t2), f.lo), f.hi),
+            fmin(fmax(fmax(t1, t2), f.lo), f.hi)};
+}
+// f_k(v) = max(m v + d, L) composed on the left of q.
+__device__ __forceinline__ Clamp compose_row(double m, double d, double L, const Clamp& q) {
+    const double t1 = fma(m, q.lo, d);
+    const double t2 = fma(m, q.hi, d);
+    return {m * q.a, fma(m, q.b, d), fmax(fmin(t1, t2), L), fmax(fmax(t1, t2), L)};
+}
+
+// Inclusive prefix over the lanes of a group, lower lanes applied first:
+// lane j ends with c_j o c_{j-1} o ... o c_0.
+template <int K>
+__device__ __forceinline__ Clamp scan_prefix(Clamp c) {
+#define MANGO_PREFIX_STEP(CTRL, ROWS)                                                             \
+    {                                                                                             \
+        const Clamp p{dpp<CTRL, ROWS>(1.0, c.a), dpp<CTRL, ROWS>(0.0, c.b), dpp<CTRL, ROWS>(-kBig, c.lo), \
+                      dpp<CTRL, ROWS>(kBig, c.hi)};                                               \
+        c = compose(c, p);                                                                        \
+    }
+    if constexpr (K > 1) MANGO_PREFIX_STEP(kRowShr + 1, 0xf)
+    if constexpr (K > 2) MANGO_PREFIX_STEP(kRowShr + 2, 0xf)
+    if constexpr (K > 4) MANGO_PREFIX_STEP(kRowShr + 4, 0xf)
+    if constexpr (K > 8) MANGO_PREFIX_STEP(kRowShr + 8, 0xf)
+    if constexpr (K > 16) MANGO_PREFIX_STEP(kRowBcast15, 0xa)
+    if constexpr (K > 32) MANGO_PREFIX_STEP(kRowBcast31, 0xc)
+#undef MANGO_PREFIX_STEP
+    return c;
+}
+
+// Inclusive suffix over the lanes of a group, higher lanes applied first:
+// lane j ends with c_j o c_{j+1} o ... o c_{K-1}.
+template <int K>
+__device__ __forceinline__ Aff scan_suffix(Aff c, int j) {
+#define MANGO_SUFFIX_STEP(D)                                                                      \
+    {                                                                                             \
+        const Aff p{dpp<kRowShl + D>(0.0, c.A), dpp<kRowShl + D>(1.0, c.B)};                      \
+        c = compose(c, p);                                                                        \
+    }
+    if constexpr (K > 1) MANGO_SUFFIX_STEP(1)
+    if constexpr (K > 2) MANGO_SUFFIX_STEP(2)
+    if constexpr (K > 4) MANGO_SUFFIX_STEP(4)
+    if constexpr (K > 8) MANGO_SUFFIX_STEP(8)
+#undef MANGO_SUFFIX_STEP
+    // rows: combine with the aggregate held by the first lane of the next row(s)
+    if constexpr (K > 16) {
+        const int src = (j & ~15) + 16;
+        const Aff p{__shfl(c.A, src, K), __shfl(c.B, src, K)};
+        if (src < K) c = compose(c, p);
+    }
+    if constexpr (K > 32) {
+        const int src = (j & ~15) + 32;
+        const Aff p{__shfl(c.A, src, K), __shfl(c.B, src, K)};
+        if (src < K) c = compose(c, p);
+    }
+    return c;
+}
+
+template <int K>
+__device__ __forceinline__ uint64_t group_lanes() {
+    if constexpr (K == 64) return ~0ull;
+    else return ((1ull << K) - 1ull) << ((__lane_id() / K) * K);
+}
+
+// ---------------------------------------------------------------------------
+// The solve kernel. Lane j of a group holds grid nodes
+// [(K-1-j) M, (K-1-j) M + M): lane K-1 holds node 0, lane 0 the top nodes
+// (and the padding beyond n-1). In this order the back substitution (from
+// node n-1 down to 0) is a prefix over lanes and the forward elimination a
+// suffix.
+// ---------------------------------------------------------------------------
 template <int K, int M>
 __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __restrict__ jobs, int n_jobs,
                                                                 const int32_t* __restrict__ d_n_jobs,
@@ -275,24 +367,25 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
     __shared__ double s_cp[GPB][NMAX];
     __shared__ double s_c[GPB][NMAX];
 
+    if (d_n_jobs) n_jobs = min(n_jobs, *d_n_jobs);
+    if (static_cast<int>(blockIdx.x) * GPB >= n_jobs) return;  // block-uniform: no barrier is skipped
     const int tid = threadIdx.x;
     const int slot = tid / K;
     const int lane = tid % K;
-    if (d_n_jobs) n_jobs = min(n_jobs, *d_n_jobs);
-    if (static_cast<int>(blockIdx.x) * GPB >= n_jobs) return;  // block-uniform: no barrier is skipped
     int job_idx = blockIdx.x * GPB + slot;
     const bool live = job_idx < n_jobs;
     if (!live) job_idx = n_jobs - 1;
     const PdeJob job = jobs[job_idx];
-    const GridDesc& g = job.grid;
-    const int n = static_cast<int>(g.n_points);
+    const int n = static_cast<int>(job.grid.n_points);
     const bool put = job.option_type == 1;
     const bool valid = params_valid(job) && n >= 3 && n <= NMAX;
-    const int base = lane * M;
+    const int base = (K - 1 - lane) * M;
+    const uint64_t gmask = group_lanes<K>();
 
     // ---- grid (GridSpec::generate) ----
     double* xs = s_x[slot];
     if (valid) {
+        const GridDesc g = job.grid;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             const int i = base + k;
@@ -300,7 +393,8 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
         }
     }
     __syncthreads();
-    if (valid && lane == 0 && needs_monotonicity(g)) enforce_monotonicity(xs, n, g.x_min, g.x_max);
+    if (valid && lane == 0 && needs_monotonicity(job.grid))
+        enforce_monotonicity(xs, n, job.grid.x_min, job.grid.x_max);
     __syncthreads();
 
     // ---- per-node constants ----
@@ -308,8 +402,7 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
     const double q = job.dividend_yield;
     const double hss = 0.5 * job.volatility * job.volatility;
     const double drift = r - q - hss;  // BlackScholesPDE::operator() (black_scholes_pde.hpp:72-76)
-    double x_k[M], psi[M], jl[M], jd[M], ju[M];
-    double c_dxl[M], c_dxr[M], c_dxc[M], c_wl[M], c_wr[M];
+    double psi[M], jl[M], jd[M], ju[M];
     uint32_t lockable = 0;
     bool nonuni = false;
     double my_psi_max = -kBig, my_dx_min = kBig;
@@ -320,10 +413,9 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
 #pragma unroll
     for (int k = 0; k < M; ++k) {
         const int i = base + k;
-        x_k[k] = 0.0; psi[k] = -kBig; jl[k] = jd[k] = ju[k] = 0.0;
-        c_dxl[k] = c_dxr[k] = c_dxc[k] = c_wl[k] = c_wr[k] = 0.0;
+        psi[k] = -kBig;
+        jl[k] = jd[k] = ju[k] = 0.0;
         if (valid && i < n) {
-            x_k[k] = xs[i];
             psi[k] = payoff(xs[i]);
             my_psi_max = smax(my_psi_max, psi[k]);
             if (i >= 1) {
@@ -332,15 +424,9 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
                 if (fabs(dx - expected_dx) > 100.0 * 2.220446049250313e-16) nonuni = true;
             }
             if (i >= 1 && i <= n - 2) {
+                // SpatialOperator::assemble_jacobian (spatial_operator.hpp:123-164)
                 const double dl = xs[i] - xs[i - 1];
                 const double dr = xs[i + 1] - xs[i];
-                const double dc = 0.5 * (dl + dr);
-                c_dxl[k] = 1.0 / dl;
-                c_dxr[k] = 1.0 / dr;
-                c_dxc[k] = 1.0 / dc;
-                c_wl[k] = dr / (dl + dr);
-                c_wr[k] = dl / (dl + dr);
-                // SpatialOperator::assemble_jacobian (spatial_operator.hpp:123-164)
                 const double a = hss, b = r - q - hss, c = -r;
                 const double dx_avg = (dl + dr) / 2.0;
                 const double d2_im1 = a / (dl * dx_avg);
@@ -356,44 +442,30 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
             }
         }
     }
-    const bool uniform = !group_any<K>(nonuni);
+    const bool uniform = (__ballot(nonuni) & gmask) == 0;
     const double psi_max = group_max<K>(my_psi_max);
     const double dx_min = group_min<K>(my_dx_min);
-    double u_dx = 0.0, u_dx_inv = 0.0, u_dx2_inv = 0.0, u_half_inv = 0.0;
-    if (uniform) {
-        u_dx = expected_dx;
-        u_dx_inv = 1.0 / u_dx;
-        u_dx2_inv = u_dx_inv * u_dx_inv;
-        u_half_inv = u_dx_inv * 0.5;
-    }
 
-    // L(v) at interior node k of this lane, given the neighbours vl/vr.
-    auto apply_L = [&](int k, double vl, double v, double vr) {
-        double d2, d1;
-        if (uniform) {
-            d2 = fma(vr + vl, u_dx2_inv, -2.0 * v * u_dx2_inv);
-            d1 = (vr - vl) * u_half_inv;
-        } else {
-            const double fwd = (vr - v) * c_dxr[k];
-            const double bwd = (v - vl) * c_dxl[k];
-            d2 = (fwd - bwd) * c_dxc[k];
-            const double t1 = c_wl[k] * (v - vl) * c_dxl[k];
-            const double t2 = c_wr[k] * (vr - v) * c_dxr[k];
-            d1 = t1 + t2;
-        }
-        return hss * d2 + drift * d1 - r * v;
-    };
-
-    // deep-ITM lock eligibility: psi > 0.95 psi_max and L(psi) < 0 (pde_solver.hpp:631-680)
+    // deep-ITM lock eligibility: psi > 0.95 psi_max and L(psi) < 0, with L
+    // evaluated by the centered stencil (pde_solver.hpp:631-680)
     {
         const double thr = kDeepItmFraction * psi_max;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             const int i = base + k;
             if (valid && psi_max > 0.0 && i >= 1 && i <= n - 2 && psi[k] > thr) {
-                const double pl = payoff(xs[i - 1]);
-                const double pr = payoff(xs[i + 1]);
-                if (apply_L(k, pl, psi[k], pr) < 0.0) lockable |= (1u << k);
+                double pv[3] = {payoff(xs[i - 1]), psi[k], payoff(xs[i + 1])};
+                double d1, d2;
+                // stencil_at reads u[i-1..i+1]: point it at a 3-value window
+                stencil_at(xs + (i - 1), pv, 3, false, 1, d1, d2);
+                if (uniform) {
+                    const double dx = expected_dx;
+                    const double dx_inv = 1.0 / dx;
+                    const double dx2_inv = dx_inv * dx_inv;
+                    d2 = fma(pv[2] + pv[0], dx2_inv, -2.0 * pv[1] * dx2_inv);
+                    d1 = (pv[2] - pv[0]) * (dx_inv * 0.5);
+                }
+                if (hss * d2 + drift * d1 - r * pv[1] < 0.0) lockable |= (1u << k);
             }
         }
     }
@@ -413,7 +485,8 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
         if (nt < job.min_time_steps) nt = job.min_time_steps;
         n_steps = nt;
     }
-    const double dt = job.maturity / static_cast<double>(n_steps);
+    const double maturity = job.maturity;
+    const double dt = maturity / static_cast<double>(n_steps);
 
     // ---- initial condition (pde_solver.hpp:96-109) ----
     double u[M], up[M];
@@ -423,62 +496,62 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
         double v = psi[k];
         if (i == 0) v = g_left;
         if (i == n - 1) v = g_right(0.0);
-        if (i >= n) v = 0.0;
         u[k] = (i < n) ? smax(v, psi[k]) : 0.0;
         up[k] = u[k];
     }
 
-    StageCache<K, M> cache[2];
-    cache[0].valid = cache[1].valid = false;
-    cache[0].mask = cache[1].mask = 0;
-    cache[0].singular = cache[1].singular = false;
+    // position of node n-1 in this lane (-1 if elsewhere); node 0 is lane K-1, k = 0
+    const int k_right = (n - 1 >= base && n - 1 < base + M) ? n - 1 - base : -1;
+
+    // c' and 1/denom per stage kind (0: Euler half step, then stage 1; 1: stage 2)
+    double cp0[M], iv0[M], cp1[M], iv1[M];
+    uint32_t cmask0 = ~0u, cmask1 = ~0u;
+    bool csing0 = false, csing1 = false;
 
     // One projected implicit stage (solve_implicit_stage_projected,
     // pde_solver.hpp:525-729): bands, Dirichlet rows, deep-ITM locks,
     // projected Thomas, boundary values. rhs/guess in, u out.
-    auto stage = [&](double t, double w, const double (&rhs)[M], const double (&guess)[M],
-                     StageCache<K, M>& cc) -> bool {
-        double lo[M], dg[M], upb[M], rb[M];
-        uint32_t mask = 0;
+    auto stage = [&](double t, double w, const double (&rhs)[M], const double (&guess)[M], double (&cp)[M],
+                     double (&iv)[M], uint32_t& cmask, bool& csing) -> bool {
         const double gr = g_right(t);
+        double nlo[M], rb[M];
+        uint32_t mask = 0;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-            const int i = base + k;
-            if (i == 0) { lo[k] = 0.0; dg[k] = 1.0; upb[k] = 0.0; rb[k] = g_left; }
-            else if (i == n - 1) { lo[k] = 0.0; dg[k] = 1.0; upb[k] = 0.0; rb[k] = gr; }
-            else if (i >= n) { lo[k] = 0.0; dg[k] = 1.0; upb[k] = 0.0; rb[k] = 0.0; }
-            else {
-                const bool lock = ((lockable >> k) & 1u) && (guess[k] - psi[k] < kExerciseTol);
-                if (lock) {
-                    mask |= (1u << k);
-                    lo[k] = 0.0; dg[k] = 1.0; upb[k] = 0.0; rb[k] = psi[k];
-                } else {
-                    lo[k] = -w * jl[k];
-                    dg[k] = 1.0 - w * jd[k];
-                    upb[k] = -w * ju[k];
-                    rb[k] = rhs[k];
-                }
-            }
+            // jl is zero off the interior, so boundary and padding rows get
+            // lower = 0 without a select; padding rows carry u = rhs = 0.
+            const bool lock = ((lockable >> k) & 1u) && (guess[k] - psi[k] < kExerciseTol);
+            if (lock) mask |= (1u << k);
+            nlo[k] = lock ? 0.0 : w * jl[k];  // -lower: -((-w) jl) == w jl
+            double v = lock ? psi[k] : rhs[k];
+            if (k == k_right) v = gr;
+            rb[k] = v;
         }
-        // --- c' and 1/denom (cached per stage kind and lock pattern) ---
-        const bool stale = group_any<K>(!cc.valid || mask != cc.mask);
-        if (stale) {
+        if (lane == K - 1) rb[0] = g_left;
+        // --- c' and 1/denom: recomputed only when the lock pattern changed ---
+        if ((__ballot(mask != cmask) & gmask) != 0) {
             double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
+            double dg[M], ub[M];
 #pragma unroll
             for (int k = 0; k < M; ++k) {
-                // T_k = [[0, up],[-l, d]] applied on the left
-                const double n00 = upb[k] * m10;
-                const double n01 = upb[k] * m11;
-                const double n10 = fma(-lo[k], m00, dg[k] * m10);
-                const double n11 = fma(-lo[k], m01, dg[k] * m11);
+                const int i = base + k;
+                const bool active = i >= 1 && i <= n - 2 && !((mask >> k) & 1u);
+                dg[k] = active ? 1.0 - w * jd[k] : 1.0;
+                ub[k] = active ? -w * ju[k] : 0.0;
+                // T_k = [[0, up], [-l, d]] on the left (node order)
+                const double n00 = ub[k] * m10;
+                const double n01 = ub[k] * m11;
+                const double n10 = fma(nlo[k], m00, dg[k] * m10);
+                const double n11 = fma(nlo[k], m01, dg[k] * m11);
                 m00 = n00; m01 = n01; m10 = n10; m11 = n11;
                 renorm(m00, m01, m10, m11);
             }
+            // suffix over lanes (node order runs from lane K-1 down to lane 0)
 #pragma unroll
             for (int d = 1; d < K; d <<= 1) {
-                const double p00 = shup<K>(m00, d), p01 = shup<K>(m01, d);
-                const double p10 = shup<K>(m10, d), p11 = shup<K>(m11, d);
-                if (lane >= d) {
+                const double p00 = shdn<K>(m00, d), p01 = shdn<K>(m01, d);
+                const double p10 = shdn<K>(m10, d), p11 = shdn<K>(m11, d);
+                if (lane + d < K) {
                     const double a00 = fma(m00, p00, m01 * p10);
                     const double a01 = fma(m00, p01, m01 * p11);
                     const double a10 = fma(m10, p00, m11 * p10);
@@ -487,89 +560,57 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
                     renorm(m00, m01, m10, m11);
                 }
             }
-            const double e01 = shup<K>(m01, 1);
-            const double e11 = shup<K>(m11, 1);
-            double cprev = (lane == 0) ? 0.0 : e01 / e11;
+            const double e01 = shdn<K>(m01, 1);
+            const double e11 = shdn<K>(m11, 1);
+            double cprev = (lane == K - 1) ? 0.0 : e01 / e11;
             bool sing = false;
 #pragma unroll
             for (int k = 0; k < M; ++k) {
-                const double denom = fma(-lo[k], cprev, dg[k]);
+                const double denom = fma(nlo[k], cprev, dg[k]);
                 if (fabs(denom) < kSingularTol) sing = true;
                 const double inv = 1.0 / denom;
-                cc.cp[k] = upb[k] * inv;
-                cc.inv[k] = inv;
-                cprev = cc.cp[k];
+                cp[k] = ub[k] * inv;
+                iv[k] = inv;
+                cprev = cp[k];
             }
-            cc.singular = group_any<K>(sing);
-            cc.mask = mask;
-            cc.valid = true;
+            csing = (__ballot(sing) & gmask) != 0;
+            cmask = mask;
         }
-        if (cc.singular) return false;
-        // --- d' : affine scan ---
-        double A = 0.0, B = 1.0;
+        if (csing) return false;
+        // --- d' = (rhs - l d'_prev) / denom: suffix scan of affine maps ---
+        Aff f{0.0, 1.0};
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-            const double al = rb[k] * cc.inv[k];
-            const double be = -lo[k] * cc.inv[k];
-            A = fma(be, A, al);
-            B = be * B;
+            const double al = rb[k] * iv[k];
+            const double be = nlo[k] * iv[k];
+            f = Aff{fma(be, f.A, al), be * f.B};
         }
-#pragma unroll
-        for (int d = 1; d < K; d <<= 1) {
-            const double pA = shup<K>(A, d), pB = shup<K>(B, d);
-            if (lane >= d) { A = fma(B, pA, A); B = B * pB; }
-        }
-        double dprev = shup<K>(A, 1);
-        if (lane == 0) dprev = 0.0;
+        f = scan_suffix<K>(f, lane);
+        double dprev = dpp<kWaveShl1>(0.0, f.A);
+        if (lane == K - 1) dprev = 0.0;
         double dp[M];
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-            dp[k] = fma(-lo[k], dprev, rb[k]) * cc.inv[k];
+            dp[k] = fma(nlo[k], dprev, rb[k]) * iv[k];
             dprev = dp[k];
         }
-        // --- projected back substitution: clamped-affine suffix scan ---
-        double fa = 1.0, fb = 0.0, flo = -kBig, fhi = kBig;
+        // --- projected back substitution: prefix scan of clamped-affine maps ---
+        Clamp F = clamp_identity();
+#pragma unroll
+        for (int k = M - 1; k >= 0; --k) F = compose_row(-cp[k], dp[k], psi[k], F);
+        F = scan_prefix<K>(F);
+        const double eb = dpp<kWaveShr1>(0.0, F.b);
+        const double elo = dpp<kWaveShr1>(-kBig, F.lo);
+        const double ehi = dpp<kWaveShr1>(kBig, F.hi);
+        double v = (lane == 0) ? 0.0 : fmin(fmax(eb, elo), ehi);
 #pragma unroll
         for (int k = M - 1; k >= 0; --k) {
-            const double mm = -cc.cp[k], dd = dp[k], L = psi[k], H = kBig;
-            const double na = mm * fa;
-            const double nb = fma(mm, fb, dd);
-            double nlo, nhi;
-            if (mm >= 0.0) { nlo = clampd(fma(mm, flo, dd), L, H); nhi = clampd(fma(mm, fhi, dd), L, H); }
-            else { nlo = clampd(fma(mm, fhi, dd), L, H); nhi = clampd(fma(mm, flo, dd), L, H); }
-            fa = na; fb = nb; flo = nlo; fhi = nhi;
+            double uk = fmax(fma(-cp[k], v, dp[k]), psi[k]);
+            v = uk;
+            if (k == k_right) uk = gr;
+            u[k] = uk;
         }
-#pragma unroll
-        for (int d = 1; d < K; d <<= 1) {
-            const double qa = shdn<K>(fa, d), qb = shdn<K>(fb, d);
-            const double qlo = shdn<K>(flo, d), qhi = shdn<K>(fhi, d);
-            if (lane + d < K) {
-                // (f o q)(v) with f = (fa, fb, flo, fhi), q = (qa, qb, qlo, qhi)
-                const double na = fa * qa;
-                const double nb = fma(fa, qb, fb);
-                double nlo, nhi;
-                if (fa >= 0.0) { nlo = clampd(fma(fa, qlo, fb), flo, fhi); nhi = clampd(fma(fa, qhi, fb), flo, fhi); }
-                else { nlo = clampd(fma(fa, qhi, fb), flo, fhi); nhi = clampd(fma(fa, qlo, fb), flo, fhi); }
-                fa = na; fb = nb; flo = nlo; fhi = nhi;
-            }
-        }
-        const double sb = shdn<K>(fb, 1), slo = shdn<K>(flo, 1), shi = shdn<K>(fhi, 1);
-        double v = (lane == K - 1) ? 0.0 : clampd(sb, slo, shi);
-        double outv[M];
-#pragma unroll
-        for (int k = M - 1; k >= 0; --k) {
-            const double unconstrained = fma(-cc.cp[k], v, dp[k]);
-            outv[k] = smax(unconstrained, psi[k]);
-            v = outv[k];
-        }
-#pragma unroll
-        for (int k = 0; k < M; ++k) {
-            const int i = base + k;
-            u[k] = outv[k];
-            if (i == 0) u[k] = g_left;
-            if (i == n - 1) u[k] = gr;
-            if (i >= n) u[k] = 0.0;
-        }
+        if (lane == K - 1) u[0] = g_left;
         return true;
     };
 
@@ -578,20 +619,20 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
     if (!live) n_steps = 0;
     if (ok && n_steps > 0) {
         // Rannacher start: two implicit-Euler half steps (pde_solver.hpp:413-447)
-        double rhs[M], guess[M];
+        double rhs[M];
         const double t0 = 0.0;
         const double t_next = t0 + dt;
         const double half_dt = 0.5 * dt;
         const double t_mid = t0 + half_dt;
 #pragma unroll
-        for (int k = 0; k < M; ++k) { up[k] = u[k]; rhs[k] = up[k]; guess[k] = up[k]; }
-        ok = stage(t_mid, half_dt, rhs, guess, cache[0]);
+        for (int k = 0; k < M; ++k) { up[k] = u[k]; rhs[k] = up[k]; }
+        ok = stage(t_mid, half_dt, rhs, up, cp0, iv0, cmask0, csing0);
         if (ok) {
 #pragma unroll
-            for (int k = 0; k < M; ++k) { up[k] = u[k]; rhs[k] = up[k]; guess[k] = up[k]; }
-            ok = stage(t_next, half_dt, rhs, guess, cache[0]);
+            for (int k = 0; k < M; ++k) { up[k] = u[k]; rhs[k] = up[k]; }
+            ok = stage(t_next, half_dt, rhs, up, cp0, iv0, cmask0, csing0);
         }
-        cache[0].valid = false;
+        cmask0 = ~0u;  // the Euler weight differs from stage 1's
     }
     if (ok) {
         const double w1 = gamma * dt / 2.0;
@@ -606,30 +647,27 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
             const double t_next = static_cast<double>(step + 1) * dt;
 #pragma unroll
             for (int k = 0; k < M; ++k) up[k] = u[k];
-            // stage 1: rhs = u^n + w1 L(u^n), guess u^n (pde_solver.hpp:342-362)
+            // stage 1: rhs = u^n + w1 L(u^n), guess u^n (pde_solver.hpp:342-362);
+            // L(u) = J u with the analytic Jacobian bands of the operator
             double rhs[M];
             {
-                const double left = shup<K>(up[M - 1], 1);
-                const double right = shdn<K>(up[0], 1);
+                const double below = dpp<kWaveShl1>(0.0, up[M - 1]);  // node base-1 (lane j+1)
+                const double above = dpp<kWaveShr1>(0.0, up[0]);      // node base+M (lane j-1)
 #pragma unroll
                 for (int k = 0; k < M; ++k) {
-                    const int i = base + k;
-                    double lu = 0.0;
-                    if (i >= 1 && i <= n - 2) {
-                        const double vl = (k == 0) ? left : up[k > 0 ? k - 1 : 0];
-                        const double vr = (k == M - 1) ? right : up[k < M - 1 ? k + 1 : M - 1];
-                        lu = apply_L(k, vl, up[k], vr);
-                    }
-                    rhs[k] = fma(w1, lu, up[k]);
+                    const double ul = (k == 0) ? below : up[k > 0 ? k - 1 : 0];
+                    const double ur = (k == M - 1) ? above : up[k < M - 1 ? k + 1 : 0];
+                    const double lu = fma(jl[k], ul, fma(ju[k], ur, jd[k] * up[k]));
+                    rhs[k] = fma(w1, lu, up[k]);  // jl = jd = ju = 0 off the interior: L = 0
                 }
             }
-            ok = stage(t + gamma * dt, w1, rhs, up, cache[0]);
+            ok = stage(t + gamma * dt, w1, rhs, up, cp0, iv0, cmask0, csing0);
             if (!ok) break;
             // stage 2: rhs = alpha u* + beta u^n, guess u* (pde_solver.hpp:370-395)
             double ustar[M];
 #pragma unroll
             for (int k = 0; k < M; ++k) { ustar[k] = u[k]; rhs[k] = fma(alpha, u[k], beta * up[k]); }
-            ok = stage(t_next, w2, rhs, ustar, cache[1]);
+            ok = stage(t_next, w2, rhs, ustar, cp1, iv1, cmask1, csing1);
             if (!ok) break;
             t = t_next;
         }
@@ -658,16 +696,13 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
     __syncthreads();
     double v_cur = 0.0;
     Greeks gk{NAN, NAN, NAN, NAN};
-    bool spl_ok = true;
     if (lane == 0 && live && ok) {
-        spl_ok = spline_second(xs, ys, n, cps, cs);
+        spline_second(xs, ys, n, cps, cs);
         const double xsp = log(job.spot / job.strike);
         v_cur = spline_eval(xs, ys, cs, n, xsp);
         if (write_state) for (int i = 0; i < n; ++i) state.c_u[row + i] = cs[i];
         if (job.flags & kJobGreeks) {
-            // value, delta, gamma need u (in ys) and its spline (in cs)
-            const bool uni = uniform;
-            Greeks t = greeks_at(xs, ys, cs, ys, cs, n, uni, job.spot, job.strike, dt);
+            const Greeks t = greeks_at(xs, ys, cs, ys, cs, n, uniform, job.spot, job.strike, dt);
             gk.value = t.value; gk.delta = t.delta; gk.gamma = t.gamma;
         }
     }
@@ -676,7 +711,7 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
     for (int k = 0; k < M; ++k) { const int i = base + k; if (i < n && valid) ys[i] = up[k]; }
     __syncthreads();
     if (lane == 0 && live && ok) {
-        spl_ok = spline_second(xs, ys, n, cps, cs) && spl_ok;
+        spline_second(xs, ys, n, cps, cs);
         if (write_state) for (int i = 0; i < n; ++i) state.c_up[row + i] = cs[i];
         if (job.flags & kJobGreeks) {
             const double xsp = log(job.spot / job.strike);
@@ -691,7 +726,6 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
         if (st != kStatusOk) gk = Greeks{NAN, NAN, NAN, NAN};
         if (job.flags & kJobGreeks) greeks_out[job.out_index] = gk;
         status_out[job.out_index] = st;
-        (void)spl_ok;
     }
 }
 
