@@ -59,6 +59,11 @@ def run_gpu(args):
     from mango_option import _abi, workloads as W
 
     lib = _abi.load_library()
+    if args.variant:
+        k, m = (int(v) for v in args.variant.split(","))
+        rule = (_abi.MangoVariantRule * 1)(_abi.MangoVariantRule(N_SPACE, k, m))
+        if lib.mango_tuning_set(rule, 1, None) != 0:
+            raise SystemExit(f"variant {args.variant} is not compiled")
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
     sp = C.c_void_p(stream.cuda_stream)
@@ -207,6 +212,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--variant", default="", help="force lanes,nodes_per_lane of the solve kernel")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

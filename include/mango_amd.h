@@ -237,6 +237,17 @@ MangoStatus mango_price_american_chains_device(const MangoDeviceOption* d_opts, 
                                                const MangoGridConfig* grid, MangoAmericanGreeks* d_out,
                                                int32_t* d_status, void* stream, MangoError* err);
 
+/* Kernel-variant selection (the autotuner's knob). A variant is (lanes per
+ * option, grid nodes per lane) and covers n_space <= lanes * nodes_per_lane.
+ * Rules are tried in ascending max_n; the first with n_space <= max_n wins,
+ * else the built-in default. Rules are also read at the first device call
+ * from $MANGO_AMD_TUNING or tuning_gfx950.txt next to the library (lines
+ * "max_n lanes nodes_per_lane"). No counterpart in the reference. */
+typedef struct { uint64_t max_n; int32_t lanes; int32_t nodes_per_lane; } MangoVariantRule;
+MangoStatus mango_tuning_set(const MangoVariantRule* rules, uint64_t n_rules, MangoError* err);
+/* Lists the compiled variants (max_n = lanes * nodes_per_lane); returns their count. */
+uint64_t mango_tuning_variants(MangoVariantRule* out, uint64_t cap);
+
 /* Benchmark instrumentation: when enabled, the device entry points bracket
  * every launch of the PDE solve kernel with HIP events on its stream;
  * mango_profile_read synchronizes them and returns the summed kernel time. */

@@ -267,6 +267,11 @@ template <int CTRL, int ROWS = 0xf>
 __device__ __forceinline__ double dpp(double old, double v) {
     return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xf, false);
 }
+// Lanes whose source is outside the row read 0 (no identity fill needed).
+template <int CTRL>
+__device__ __forceinline__ double dpp0(double v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
+}
 
 // Affine map x -> A + B x (forward-elimination recurrence of d').
 struct Aff {
@@ -293,6 +298,18 @@ __device__ __forceinline__ Clamp compose_row(double m, double d, double L, const
     return {m * q.a, fma(m, q.b, d), fmax(fmin(t1, t2), L), fmax(fmax(t1, t2), L)};
 }
 
+// Non-decreasing clamp v -> max(a v + b, lo), a >= 0: the form every row map
+// takes when all c' <= 0 (the usual case: then -c' >= 0).
+struct Mono {
+    double a, b, lo;
+};
+__device__ __forceinline__ Mono compose(const Mono& f, const Mono& q) {
+    return {f.a * q.a, fma(f.a, q.b, f.b), fmax(fma(f.a, q.lo, f.b), f.lo)};
+}
+__device__ __forceinline__ Mono compose_row_mono(double m, double d, double L, const Mono& q) {
+    return {m * q.a, fma(m, q.b, d), fmax(fma(m, q.lo, d), L)};
+}
+
 // Inclusive prefix over the lanes of a group, lower lanes applied first:
 // lane j ends with c_j o c_{j-1} o ... o c_0.
 template <int K>
@@ -313,32 +330,72 @@ __device__ __forceinline__ Clamp scan_prefix(Clamp c) {
     return c;
 }
 
-// Inclusive suffix over the lanes of a group, higher lanes applied first:
-// lane j ends with c_j o c_{j+1} o ... o c_{K-1}.
+// Same for monotone maps. With K = 16 the whole group is one DPP row, and a
+// lane whose in-row source does not exist already holds its complete prefix,
+// which is a constant map (it contains the top node, where c' = 0): composing
+// it with the zero map read by bound_ctrl keeps its value at every v.
 template <int K>
-__device__ __forceinline__ Aff scan_suffix(Aff c, int j) {
-#define MANGO_SUFFIX_STEP(D)                                                                      \
-    {                                                                                             \
-        const Aff p{dpp<kRowShl + D>(0.0, c.A), dpp<kRowShl + D>(1.0, c.B)};                      \
-        c = compose(c, p);                                                                        \
-    }
-    if constexpr (K > 1) MANGO_SUFFIX_STEP(1)
-    if constexpr (K > 2) MANGO_SUFFIX_STEP(2)
-    if constexpr (K > 4) MANGO_SUFFIX_STEP(4)
-    if constexpr (K > 8) MANGO_SUFFIX_STEP(8)
-#undef MANGO_SUFFIX_STEP
-    // rows: combine with the aggregate held by the first lane of the next row(s)
-    if constexpr (K > 16) {
-        const int src = (j & ~15) + 16;
-        const Aff p{__shfl(c.A, src, K), __shfl(c.B, src, K)};
-        if (src < K) c = compose(c, p);
-    }
-    if constexpr (K > 32) {
-        const int src = (j & ~15) + 32;
-        const Aff p{__shfl(c.A, src, K), __shfl(c.B, src, K)};
-        if (src < K) c = compose(c, p);
+__device__ __forceinline__ Mono scan_prefix(Mono c) {
+    if constexpr (K == 16) {
+#define MANGO_STEP0(D) { const Mono p{dpp0<kRowShr + D>(c.a), dpp0<kRowShr + D>(c.b), dpp0<kRowShr + D>(c.lo)}; c = compose(c, p); }
+        MANGO_STEP0(1) MANGO_STEP0(2) MANGO_STEP0(4) MANGO_STEP0(8)
+#undef MANGO_STEP0
+    } else {
+#define MANGO_STEP(CTRL, ROWS) { const Mono p{dpp<CTRL, ROWS>(1.0, c.a), dpp<CTRL, ROWS>(0.0, c.b), dpp<CTRL, ROWS>(-kBig, c.lo)}; c = compose(c, p); }
+        if constexpr (K > 1) MANGO_STEP(kRowShr + 1, 0xf)
+        if constexpr (K > 2) MANGO_STEP(kRowShr + 2, 0xf)
+        if constexpr (K > 4) MANGO_STEP(kRowShr + 4, 0xf)
+        if constexpr (K > 8) MANGO_STEP(kRowShr + 8, 0xf)
+        if constexpr (K > 16) MANGO_STEP(kRowBcast15, 0xa)
+        if constexpr (K > 32) MANGO_STEP(kRowBcast31, 0xc)
+#undef MANGO_STEP
     }
     return c;
+}
+
+// Inclusive suffix over the lanes of a group, higher lanes applied first:
+// lane j ends with c_j o c_{j+1} o ... o c_{K-1}.
+// Value of a double in wavefront lane L (uniform result).
+__device__ __forceinline__ double readlane_d(double v, int L) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffll), L);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), L);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+template <int K>
+__device__ __forceinline__ Aff scan_suffix(Aff c, int j) {
+    if constexpr (K == 16) {
+        // out-of-row lanes read (0, 0); such a lane already holds its complete
+        // suffix, which contains node 0 (beta = 0), so its B is 0 and A is kept.
+#define MANGO_STEP0(D) { const Aff p{dpp0<kRowShl + D>(c.A), dpp0<kRowShl + D>(c.B)}; c = compose(c, p); }
+        MANGO_STEP0(1) MANGO_STEP0(2) MANGO_STEP0(4) MANGO_STEP0(8)
+#undef MANGO_STEP0
+        return c;
+    } else {
+#define MANGO_STEP(D) { const Aff p{dpp<kRowShl + D>(0.0, c.A), dpp<kRowShl + D>(1.0, c.B)}; c = compose(c, p); }
+        MANGO_STEP(1) MANGO_STEP(2) MANGO_STEP(4) MANGO_STEP(8)
+#undef MANGO_STEP
+        // Row crossing: the first lane of each row now holds its row's
+        // aggregate; read them as scalars and give each row the composition
+        // of all rows after it within the group.
+        const int wl = __lane_id();
+        if constexpr (K == 32) {
+            const Aff a1{readlane_d(c.A, 16), readlane_d(c.B, 16)};
+            const Aff a3{readlane_d(c.A, 48), readlane_d(c.B, 48)};
+            const Aff nxt = (wl < 32) ? a1 : a3;
+            if ((j & 16) == 0) c = compose(c, nxt);
+        } else {
+            const Aff a1{readlane_d(c.A, 16), readlane_d(c.B, 16)};
+            const Aff a2{readlane_d(c.A, 32), readlane_d(c.B, 32)};
+            const Aff a3{readlane_d(c.A, 48), readlane_d(c.B, 48)};
+            const Aff s2 = compose(a2, a3);
+            const Aff s1 = compose(a1, s2);
+            const int row = j >> 4;
+            if (row < 3) c = compose(c, row == 0 ? s1 : (row == 1 ? s2 : a3));
+        }
+        return c;
+    }
 }
 
 template <int K>
@@ -507,12 +564,13 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
     double cp0[M], iv0[M], cp1[M], iv1[M];
     uint32_t cmask0 = ~0u, cmask1 = ~0u;
     bool csing0 = false, csing1 = false;
+    bool cmono0 = false, cmono1 = false;
 
     // One projected implicit stage (solve_implicit_stage_projected,
     // pde_solver.hpp:525-729): bands, Dirichlet rows, deep-ITM locks,
     // projected Thomas, boundary values. rhs/guess in, u out.
     auto stage = [&](double t, double w, const double (&rhs)[M], const double (&guess)[M], double (&cp)[M],
-                     double (&iv)[M], uint32_t& cmask, bool& csing) -> bool {
+                     double (&iv)[M], uint32_t& cmask, bool& csing, bool& cmono) -> bool {
         const double gr = g_right(t);
         double nlo[M], rb[M];
         uint32_t mask = 0;
@@ -574,6 +632,10 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
                 cprev = cp[k];
             }
             csing = (__ballot(sing) & gmask) != 0;
+            bool pos = false;
+#pragma unroll
+            for (int k = 0; k < M; ++k) pos = pos || (cp[k] > 0.0);
+            cmono = (__ballot(pos) & gmask) == 0;
             cmask = mask;
         }
         if (csing) return false;
@@ -595,14 +657,25 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
             dprev = dp[k];
         }
         // --- projected back substitution: prefix scan of clamped-affine maps ---
-        Clamp F = clamp_identity();
+        double v;
+        if (cmono) {
+            Mono F{1.0, 0.0, -kBig};
 #pragma unroll
-        for (int k = M - 1; k >= 0; --k) F = compose_row(-cp[k], dp[k], psi[k], F);
-        F = scan_prefix<K>(F);
-        const double eb = dpp<kWaveShr1>(0.0, F.b);
-        const double elo = dpp<kWaveShr1>(-kBig, F.lo);
-        const double ehi = dpp<kWaveShr1>(kBig, F.hi);
-        double v = (lane == 0) ? 0.0 : fmin(fmax(eb, elo), ehi);
+            for (int k = M - 1; k >= 0; --k) F = compose_row_mono(-cp[k], dp[k], psi[k], F);
+            F = scan_prefix<K>(F);
+            const double eb = dpp<kWaveShr1>(0.0, F.b);
+            const double elo = dpp<kWaveShr1>(-kBig, F.lo);
+            v = (lane == 0) ? 0.0 : fmax(eb, elo);
+        } else {
+            Clamp F = clamp_identity();
+#pragma unroll
+            for (int k = M - 1; k >= 0; --k) F = compose_row(-cp[k], dp[k], psi[k], F);
+            F = scan_prefix<K>(F);
+            const double eb = dpp<kWaveShr1>(0.0, F.b);
+            const double elo = dpp<kWaveShr1>(-kBig, F.lo);
+            const double ehi = dpp<kWaveShr1>(kBig, F.hi);
+            v = (lane == 0) ? 0.0 : fmin(fmax(eb, elo), ehi);
+        }
 #pragma unroll
         for (int k = M - 1; k >= 0; --k) {
             double uk = fmax(fma(-cp[k], v, dp[k]), psi[k]);
@@ -626,11 +699,11 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
         const double t_mid = t0 + half_dt;
 #pragma unroll
         for (int k = 0; k < M; ++k) { up[k] = u[k]; rhs[k] = up[k]; }
-        ok = stage(t_mid, half_dt, rhs, up, cp0, iv0, cmask0, csing0);
+        ok = stage(t_mid, half_dt, rhs, up, cp0, iv0, cmask0, csing0, cmono0);
         if (ok) {
 #pragma unroll
             for (int k = 0; k < M; ++k) { up[k] = u[k]; rhs[k] = up[k]; }
-            ok = stage(t_next, half_dt, rhs, up, cp0, iv0, cmask0, csing0);
+            ok = stage(t_next, half_dt, rhs, up, cp0, iv0, cmask0, csing0, cmono0);
         }
         cmask0 = ~0u;  // the Euler weight differs from stage 1's
     }
@@ -661,13 +734,13 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
                     rhs[k] = fma(w1, lu, up[k]);  // jl = jd = ju = 0 off the interior: L = 0
                 }
             }
-            ok = stage(t + gamma * dt, w1, rhs, up, cp0, iv0, cmask0, csing0);
+            ok = stage(t + gamma * dt, w1, rhs, up, cp0, iv0, cmask0, csing0, cmono0);
             if (!ok) break;
             // stage 2: rhs = alpha u* + beta u^n, guess u* (pde_solver.hpp:370-395)
             double ustar[M];
 #pragma unroll
             for (int k = 0; k < M; ++k) { ustar[k] = u[k]; rhs[k] = fma(alpha, u[k], beta * up[k]); }
-            ok = stage(t_next, w2, rhs, ustar, cp1, iv1, cmask1, csing1);
+            ok = stage(t_next, w2, rhs, ustar, cp1, iv1, cmask1, csing1, cmono1);
             if (!ok) break;
             t = t_next;
         }
@@ -761,23 +834,26 @@ __global__ __launch_bounds__(kBlock) void chain_epilogue_kernel(const ChainOptio
 // 219-270) and one S = K = 1 PDE is emitted per group. On one shared grid the
 // reference's regular route gives each member the same u(x) (the PDE is
 // strike-free), so deduplicating is exact for either route.
-__global__ __launch_bounds__(kBlock) void plan_chains_kernel(const DeviceOptionRecord* __restrict__ opts,
-                                                             const int64_t* __restrict__ seg_offsets, int n_segments,
-                                                             DevicePlan plan, PdeJob* __restrict__ jobs,
-                                                             int32_t* __restrict__ job_count,
-                                                             ChainOption* __restrict__ copts,
-                                                             int32_t* __restrict__ rep) {
-    const int sg = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void plan_chains_kernel(const DeviceOptionRecord* __restrict__ opts,
+                                                         const int64_t* __restrict__ seg_offsets, int n_segments,
+                                                         DevicePlan plan, PdeJob* __restrict__ jobs,
+                                                         int32_t* __restrict__ job_count,
+                                                         ChainOption* __restrict__ copts,
+                                                         int32_t* __restrict__ rep) {
+    // one wavefront per segment
+    const int sg = blockIdx.x;
+    const int lane = threadIdx.x;
     if (sg >= n_segments) return;
     const int64_t b = seg_offsets[sg], e = seg_offsets[sg + 1];
     if (e <= b) return;
-    double gx_min = 1.7976931348623157e308, gx_max = -1.7976931348623157e308;
-    int64_t gnx = 0;
     auto valid = [](const DeviceOptionRecord& o) {
         return o.spot > 0.0 && o.strike > 0.0 && o.maturity > 0.0 && o.volatility > 0.0 && isfinite(o.spot) &&
                isfinite(o.strike) && isfinite(o.maturity) && isfinite(o.volatility);
     };
-    for (int64_t i = b; i < e; ++i) {
+    // spatial union of estimate_batch_pde_grid: lanes stride over the options
+    double gx_min = 1.7976931348623157e308, gx_max = -1.7976931348623157e308;
+    int64_t gnx = 0;
+    for (int64_t i = b + lane; i < e; i += 64) {
         const DeviceOptionRecord o = opts[i];
         if (!valid(o)) continue;
         const GridDesc g = estimate_spatial(o.spot, o.strike, o.maturity, o.volatility, plan.accuracy);
@@ -785,56 +861,66 @@ __global__ __launch_bounds__(kBlock) void plan_chains_kernel(const DeviceOptionR
         gx_max = smax(gx_max, g.x_max);
         gnx = gnx > g.n_points ? gnx : g.n_points;
     }
-    GridDesc shared;
-    if (plan.mode == 1) {
-        shared = plan.spec;
-    } else {
-        const double c[1] = {0.0}, a[1] = {plan.accuracy.alpha}, w[1] = {1.0};
-        if (!make_multi_sinh(shared, gx_min, gx_max, gnx, c, a, w, 1)) {
-            shared = GridDesc{};
-            shared.type = kSinh; shared.x_min = gx_min; shared.x_max = gx_max; shared.n_points = gnx;
-            shared.concentration = plan.accuracy.alpha; shared.n_clusters = 0;
-        }
+    gx_min = group_min<64>(gx_min);
+    gx_max = group_max<64>(gx_max);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int64_t other = __shfl_xor(gnx, o, 64);
+        gnx = gnx > other ? gnx : other;
     }
-    // greedy first-match grouping; rep[i] = index of the group's first member
+    // greedy first-match grouping on one lane (order-dependent, like the reference)
     constexpr double TOL = 1e-10;
-    int32_t n_groups = 0;
-    for (int64_t i = b; i < e; ++i) {
-        const DeviceOptionRecord o = opts[i];
-        int32_t r = static_cast<int32_t>(i);
-        for (int64_t j = b; j < i; ++j) {
-            if (rep[j] != j) continue;
-            const DeviceOptionRecord p = opts[j];
-            if (fabs(p.volatility - o.volatility) < TOL && fabs(p.rate - o.rate) < TOL &&
-                fabs(p.dividend_yield - o.dividend_yield) < TOL && fabs(p.maturity - o.maturity) < TOL &&
-                p.option_type == o.option_type) {
-                r = static_cast<int32_t>(j);
-                break;
+    if (lane == 0) {
+        GridDesc shared;
+        if (plan.mode == 1) {
+            shared = plan.spec;
+        } else {
+            const double c[1] = {0.0}, a[1] = {plan.accuracy.alpha}, w[1] = {1.0};
+            if (!make_multi_sinh(shared, gx_min, gx_max, gnx, c, a, w, 1)) {
+                shared = GridDesc{};
+                shared.type = kSinh; shared.x_min = gx_min; shared.x_max = gx_max; shared.n_points = gnx;
+                shared.concentration = plan.accuracy.alpha; shared.n_clusters = 0;
             }
         }
-        rep[i] = r;
-        if (r == i) ++n_groups;
+        int32_t n_groups = 0;
+        for (int64_t i = b; i < e; ++i) {
+            const DeviceOptionRecord o = opts[i];
+            int32_t r = static_cast<int32_t>(i);
+            for (int64_t j = b; j < i; ++j) {
+                if (rep[j] != j) continue;
+                const DeviceOptionRecord p = opts[j];
+                if (fabs(p.volatility - o.volatility) < TOL && fabs(p.rate - o.rate) < TOL &&
+                    fabs(p.dividend_yield - o.dividend_yield) < TOL && fabs(p.maturity - o.maturity) < TOL &&
+                    p.option_type == o.option_type) {
+                    r = static_cast<int32_t>(j);
+                    break;
+                }
+            }
+            rep[i] = r;
+            if (r == i) ++n_groups;
+        }
+        const int32_t base = atomicAdd(job_count, n_groups);
+        int32_t ord = 0;
+        for (int64_t i = b; i < e; ++i) {
+            if (rep[i] != i) continue;
+            const DeviceOptionRecord o = opts[i];
+            PdeJob j;
+            j.maturity = o.maturity; j.volatility = o.volatility; j.rate = o.rate; j.dividend_yield = o.dividend_yield;
+            j.spot = 1.0; j.strike = 1.0; j.c_t = plan.accuracy.c_t; j.pad0 = 0.0;
+            j.grid = shared;
+            j.option_type = o.option_type;
+            j.n_time = plan.n_time;
+            j.max_time_steps = plan.accuracy.max_time_steps;
+            j.min_time_steps = 0;
+            j.out_index = base + ord;
+            j.flags = kJobWriteState;
+            jobs[base + ord] = j;
+            rep[i] = -(base + ord) - 1;  // representative: its group slot
+            ++ord;
+        }
     }
-    const int32_t base = atomicAdd(job_count, n_groups);
-    int32_t ord = 0;
-    for (int64_t i = b; i < e; ++i) {
-        if (rep[i] != i) continue;
-        const DeviceOptionRecord o = opts[i];
-        PdeJob j;
-        j.maturity = o.maturity; j.volatility = o.volatility; j.rate = o.rate; j.dividend_yield = o.dividend_yield;
-        j.spot = 1.0; j.strike = 1.0; j.c_t = plan.accuracy.c_t; j.pad0 = 0.0;
-        j.grid = shared;
-        j.option_type = o.option_type;
-        j.n_time = plan.n_time;
-        j.max_time_steps = plan.accuracy.max_time_steps;
-        j.min_time_steps = 0;
-        j.out_index = base + ord;
-        j.flags = kJobWriteState;
-        jobs[base + ord] = j;
-        rep[i] = -(base + ord) - 1;  // mark: group slot of this representative
-        ++ord;
-    }
-    for (int64_t i = b; i < e; ++i) {
+    __syncthreads();
+    for (int64_t i = b + lane; i < e; i += 64) {
         const int32_t r = rep[i];
         const int32_t slot = r < 0 ? -r - 1 : -rep[r] - 1;
         ChainOption c;
@@ -894,14 +980,51 @@ int launch_variant(const PdeJob* d_jobs, int n_jobs, const int32_t* d_n_jobs, Gr
 
 }  // namespace
 
+// Compiled (lanes, nodes-per-lane) variants. Each covers n <= lanes * nodes.
+#define MANGO_VARIANTS(X) X(16, 4) X(16, 7) X(16, 8) X(32, 2) X(32, 4) X(32, 8) X(64, 2) X(64, 4) X(64, 7)
+
 bool variant_supported(const SolveVariant& v) {
-    return (v.lanes == 64 && (v.nodes == 2 || v.nodes == 4 || v.nodes == 7)) ||
-           (v.lanes == 32 && v.nodes == 4) || (v.lanes == 16 && v.nodes == 8);
+#define MANGO_IS(K, M) if (v.lanes == K && v.nodes == M) return true;
+    MANGO_VARIANTS(MANGO_IS)
+#undef MANGO_IS
+    return false;
+}
+
+int list_variants(SolveVariant* out, int cap) {
+    int c = 0;
+#define MANGO_LIST(K, M) { if (c < cap) out[c] = SolveVariant{K, M}; ++c; }
+    MANGO_VARIANTS(MANGO_LIST)
+#undef MANGO_LIST
+    return c;
 }
 
 int max_nodes_supported() { return 64 * 7; }
 
+namespace {
+struct Rule {
+    int64_t max_n;
+    SolveVariant v;
+};
+Rule g_rules[16];
+int g_n_rules = 0;
+}  // namespace
+
+void set_variant_rules(const int64_t* max_n, const SolveVariant* v, int n) {
+    g_n_rules = 0;
+    for (int i = 0; i < n && g_n_rules < 16; ++i) {
+        if (!variant_supported(v[i]) || max_n[i] > static_cast<int64_t>(v[i].lanes) * v[i].nodes) continue;
+        g_rules[g_n_rules++] = Rule{max_n[i], v[i]};
+    }
+    // keep ascending max_n
+    for (int i = 1; i < g_n_rules; ++i)
+        for (int j = i; j > 0 && g_rules[j].max_n < g_rules[j - 1].max_n; --j) {
+            const Rule t = g_rules[j]; g_rules[j] = g_rules[j - 1]; g_rules[j - 1] = t;
+        }
+}
+
 SolveVariant pick_variant(int64_t n_space) {
+    for (int i = 0; i < g_n_rules; ++i)
+        if (n_space <= g_rules[i].max_n) return g_rules[i].v;
     if (n_space <= 128) return {32, 4};
     if (n_space <= 256) return {64, 4};
     return {64, 7};
@@ -910,11 +1033,10 @@ SolveVariant pick_variant(int64_t n_space) {
 int launch_american_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, Greeks* d_greeks,
                           int32_t* d_status, const StateOut& state, hipStream_t stream, const int32_t* d_n_jobs) {
     if (n_jobs <= 0) return 0;
-    if (v.lanes == 64 && v.nodes == 2) return launch_variant<64, 2>(d_jobs, n_jobs, d_n_jobs, d_greeks, d_status, state, stream);
-    if (v.lanes == 64 && v.nodes == 4) return launch_variant<64, 4>(d_jobs, n_jobs, d_n_jobs, d_greeks, d_status, state, stream);
-    if (v.lanes == 64 && v.nodes == 7) return launch_variant<64, 7>(d_jobs, n_jobs, d_n_jobs, d_greeks, d_status, state, stream);
-    if (v.lanes == 32 && v.nodes == 4) return launch_variant<32, 4>(d_jobs, n_jobs, d_n_jobs, d_greeks, d_status, state, stream);
-    if (v.lanes == 16 && v.nodes == 8) return launch_variant<16, 8>(d_jobs, n_jobs, d_n_jobs, d_greeks, d_status, state, stream);
+#define MANGO_LAUNCH(K, M) \
+    if (v.lanes == K && v.nodes == M) return launch_variant<K, M>(d_jobs, n_jobs, d_n_jobs, d_greeks, d_status, state, stream);
+    MANGO_VARIANTS(MANGO_LAUNCH)
+#undef MANGO_LAUNCH
     return 2;
 }
 
@@ -930,7 +1052,7 @@ int launch_plan_chains(const DeviceOptionRecord* d_opts, const int64_t* d_seg_of
                        const DevicePlan& plan, PdeJob* d_jobs, int32_t* d_job_count, ChainOption* d_copts,
                        int32_t* d_rep, hipStream_t stream) {
     if (n_segments <= 0) return 0;
-    hipLaunchKernelGGL(plan_chains_kernel, dim3((n_segments + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d_opts,
+    hipLaunchKernelGGL(plan_chains_kernel, dim3(n_segments), dim3(64), 0, stream, d_opts,
                        d_seg_offsets, n_segments, plan, d_jobs, d_job_count, d_copts, d_rep);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }

@@ -69,7 +69,10 @@ int launch_american_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_job
                           Greeks* d_greeks, int32_t* d_status, const StateOut& state,
                           hipStream_t stream, const int32_t* d_n_jobs = nullptr);
 bool variant_supported(const SolveVariant& v);
+int list_variants(SolveVariant* out, int cap);
 int max_nodes_supported();
+// Variant choice: the first rule with n_space <= max_n, else the built-in default.
+void set_variant_rules(const int64_t* max_n, const SolveVariant* v, int n);
 SolveVariant pick_variant(int64_t n_space);
 
 // Per-option epilogue of the normalized chain: Greeks of option i from the

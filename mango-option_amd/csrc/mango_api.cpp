@@ -14,9 +14,13 @@
 //   mango_price_american             src/ffi/mango_c_api.cpp:309-361
 #include "../../include/mango_amd.h"
 
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -180,9 +184,40 @@ struct DevBuf {
 int timed_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, Greeks* d_greeks, int32_t* d_status,
                 const StateOut& state, hipStream_t stream, const int32_t* d_n_jobs);
 
+// Variant rules "max_n lanes nodes" per line ('#' starts a comment), read
+// from $MANGO_AMD_TUNING or tuning_gfx950.txt next to this library.
+void load_tuning_file() {
+    std::string path;
+    if (const char* env = std::getenv("MANGO_AMD_TUNING")) {
+        path = env;
+    } else {
+        Dl_info info;
+        if (dladdr(reinterpret_cast<void*>(&load_tuning_file), &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuning_gfx950.txt";
+        }
+    }
+    std::ifstream f(path);
+    if (!f) return;
+    std::vector<int64_t> mx;
+    std::vector<SolveVariant> vs;
+    std::string line;
+    while (std::getline(f, line)) {
+        const size_t h = line.find('#');
+        if (h != std::string::npos) line.resize(h);
+        std::istringstream is(line);
+        int64_t m;
+        int k, nodes;
+        if (is >> m >> k >> nodes) { mx.push_back(m); vs.push_back({k, nodes}); }
+    }
+    set_variant_rules(mx.data(), vs.data(), static_cast<int>(mx.size()));
+}
+
 bool device_ready(MangoError* err) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_device_ok < 0) {
+        load_tuning_file();
         int count = 0;
         if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
             g_device_ok = 0;
@@ -939,6 +974,32 @@ MangoStatus mango_price_american_chains_device(const MangoDeviceOption* d_opts, 
                                                int32_t* d_status, void* stream, MangoError* err) {
     return device_chains(d_opts, n, d_seg_offsets, n_segments, grid, d_out, d_status,
                          static_cast<hipStream_t>(stream), err);
+}
+
+MangoStatus mango_tuning_set(const MangoVariantRule* rules, uint64_t n_rules, MangoError* err) {
+    if (n_rules && !rules) { set_err(err, MANGO_ERR_VALIDATION, "null rules"); return MANGO_ERR_VALIDATION; }
+    std::vector<int64_t> mx;
+    std::vector<SolveVariant> vs;
+    for (uint64_t i = 0; i < n_rules; ++i) {
+        const SolveVariant v{rules[i].lanes, rules[i].nodes_per_lane};
+        if (!variant_supported(v) || rules[i].max_n > static_cast<uint64_t>(v.lanes) * v.nodes) {
+            set_err(err, MANGO_ERR_VALIDATION, "rule names a variant that is not compiled or too small");
+            return MANGO_ERR_VALIDATION;
+        }
+        mx.push_back(static_cast<int64_t>(rules[i].max_n));
+        vs.push_back(v);
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    set_variant_rules(mx.data(), vs.data(), static_cast<int>(mx.size()));
+    return MANGO_OK;
+}
+
+uint64_t mango_tuning_variants(MangoVariantRule* out, uint64_t cap) {
+    SolveVariant v[32];
+    const int c = list_variants(v, 32);
+    for (int i = 0; i < c && static_cast<uint64_t>(i) < cap; ++i)
+        out[i] = MangoVariantRule{static_cast<uint64_t>(v[i].lanes) * v[i].nodes, v[i].lanes, v[i].nodes};
+    return static_cast<uint64_t>(c);
 }
 
 void mango_profile_enable(int32_t enable) {

@@ -103,6 +103,10 @@ class MangoAmericanGreeks(C.Structure):
     _fields_ = [("value", C.c_double), ("delta", C.c_double), ("gamma", C.c_double), ("theta", C.c_double)]
 
 
+class MangoVariantRule(C.Structure):
+    _fields_ = [("max_n", C.c_uint64), ("lanes", C.c_int32), ("nodes_per_lane", C.c_int32)]
+
+
 class MangoDeviceOption(C.Structure):
     _fields_ = [
         ("spot", C.c_double),
@@ -162,6 +166,8 @@ SIGNATURES = {
     "mango_price_american_chains_device": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
                                                        _P(MangoGridConfig), C.c_void_p, C.c_void_p, C.c_void_p,
                                                        _P(MangoError)]),
+    "mango_tuning_set": (C.c_int32, [_P(MangoVariantRule), C.c_uint64, _P(MangoError)]),
+    "mango_tuning_variants": (C.c_uint64, [_P(MangoVariantRule), C.c_uint64]),
     "mango_profile_enable": (None, [C.c_int32]),
     "mango_profile_read": (C.c_int32, [_P(C.c_double), _P(C.c_uint64)]),
 }
