@@ -237,6 +237,20 @@ MangoStatus mango_price_american_chains_device(const MangoDeviceOption* d_opts, 
                                                const MangoGridConfig* grid, MangoAmericanGreeks* d_out,
                                                int32_t* d_status, void* stream, MangoError* err);
 
+/* Batched form with inputs in HBM: options of uneven grid sizes are binned
+ * by grid size on the device (one solve-kernel variant per bin), ordered by
+ * estimated cost (longest first) and served by persistent work-queue kernels,
+ * one per bin, running side by side. The whole sequence is captured into a
+ * hipGraph on the first call for a given (buffers, n, grid, stream) and
+ * replayed afterwards. Results equal mango_price_american_batch_device's.
+ * queue_cfg may be NULL (persistent work queues, graph replay). Calls share
+ * the library's device scratch: do not overlap calls. */
+typedef struct { int32_t persistent; int32_t use_graph; } MangoQueueConfig;
+MangoStatus mango_price_american_queue_device(const MangoDeviceOption* d_opts, uint64_t n,
+                                              const MangoGridConfig* grid, MangoAmericanGreeks* d_out,
+                                              int32_t* d_status, void* stream, const MangoQueueConfig* queue_cfg,
+                                              MangoError* err);
+
 /* Kernel-variant selection (the autotuner's knob). A variant is (lanes per
  * option, grid nodes per lane) and covers n_space <= lanes * nodes_per_lane.
  * Rules are tried in ascending max_n; the first with n_space <= max_n wins,

@@ -28,6 +28,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -35,6 +36,7 @@ namespace mango_amd {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kSolveBlock = 64;  // one wavefront per block
 constexpr double kBig = 1e300;
 constexpr double kSingularTol = 1e-15;   // ThomasConfig::singularity_tol
 constexpr double kDeepItmFraction = 0.95; // pde_solver.hpp:635
@@ -411,28 +413,22 @@ __device__ __forceinline__ uint64_t group_lanes() {
 // node n-1 down to 0) is a prefix over lanes and the forward elimination a
 // suffix.
 // ---------------------------------------------------------------------------
-template <int K, int M>
-__global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __restrict__ jobs, int n_jobs,
-                                                                const int32_t* __restrict__ d_n_jobs,
-                                                                Greeks* __restrict__ greeks_out,
-                                                                int32_t* __restrict__ status_out,
-                                                                StateOut state) {
-    constexpr int NMAX = K * M;
-    constexpr int GPB = kBlock / K;
-    __shared__ double s_x[GPB][NMAX];
-    __shared__ double s_y[GPB][NMAX];
-    __shared__ double s_cp[GPB][NMAX];
-    __shared__ double s_c[GPB][NMAX];
+// Wave-scope ordering of LDS traffic: a group never spans two wavefronts.
+__device__ __forceinline__ void group_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
-    if (d_n_jobs) n_jobs = min(n_jobs, *d_n_jobs);
-    if (static_cast<int>(blockIdx.x) * GPB >= n_jobs) return;  // block-uniform: no barrier is skipped
-    const int tid = threadIdx.x;
-    const int slot = tid / K;
-    const int lane = tid % K;
-    int job_idx = blockIdx.x * GPB + slot;
-    const bool live = job_idx < n_jobs;
-    if (!live) job_idx = n_jobs - 1;
-    const PdeJob job = jobs[job_idx];
+// One option, start to finish, by the K lanes of a group. xs/ys/cps/cs are
+// the group's LDS rows (NMAX doubles each).
+template <int K, int M>
+__device__ __forceinline__ void solve_one(const PdeJob* __restrict__ jobs, int job_index, const int lane, double* __restrict__ xs,
+                                          double* __restrict__ ys, double* __restrict__ cps, double* __restrict__ cs,
+                                          Greeks* __restrict__ greeks_out, int32_t* __restrict__ status_out,
+                                          const StateOut& state) {
+    constexpr int NMAX = K * M;
+    const PdeJob job = jobs[job_index];
     const int n = static_cast<int>(job.grid.n_points);
     const bool put = job.option_type == 1;
     const bool valid = params_valid(job) && n >= 3 && n <= NMAX;
@@ -440,7 +436,6 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
     const uint64_t gmask = group_lanes<K>();
 
     // ---- grid (GridSpec::generate) ----
-    double* xs = s_x[slot];
     if (valid) {
         const GridDesc g = job.grid;
 #pragma unroll
@@ -449,10 +444,10 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
             if (i < n) xs[i] = raw_grid_point(g, i);
         }
     }
-    __syncthreads();
+    group_sync();
     if (valid && lane == 0 && needs_monotonicity(job.grid))
         enforce_monotonicity(xs, n, job.grid.x_min, job.grid.x_max);
-    __syncthreads();
+    group_sync();
 
     // ---- per-node constants ----
     const double r = job.rate;
@@ -689,7 +684,6 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
 
     bool ok = valid;
     const double gamma = 2.0 - sqrt(2.0);
-    if (!live) n_steps = 0;
     if (ok && n_steps > 0) {
         // Rannacher start: two implicit-Euler half steps (pde_solver.hpp:413-447)
         double rhs[M];
@@ -747,10 +741,7 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
     }
 
     // ---- epilogue: natural splines and Greeks on one lane ----
-    double* ys = s_y[slot];
-    double* cps = s_cp[slot];
-    double* cs = s_c[slot];
-    const bool write_state = live && (job.flags & kJobWriteState) && state.u != nullptr;
+    const bool write_state = (job.flags & kJobWriteState) && state.u != nullptr;
     const int64_t row = static_cast<int64_t>(job.out_index) * state.stride;
     if (write_state && ok) {
 #pragma unroll
@@ -766,10 +757,10 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
     }
 #pragma unroll
     for (int k = 0; k < M; ++k) { const int i = base + k; if (i < n && valid) ys[i] = u[k]; }
-    __syncthreads();
+    group_sync();
     double v_cur = 0.0;
     Greeks gk{NAN, NAN, NAN, NAN};
-    if (lane == 0 && live && ok) {
+    if (lane == 0 && ok) {
         spline_second(xs, ys, n, cps, cs);
         const double xsp = log(job.spot / job.strike);
         v_cur = spline_eval(xs, ys, cs, n, xsp);
@@ -779,11 +770,11 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
             gk.value = t.value; gk.delta = t.delta; gk.gamma = t.gamma;
         }
     }
-    __syncthreads();
+    group_sync();
 #pragma unroll
     for (int k = 0; k < M; ++k) { const int i = base + k; if (i < n && valid) ys[i] = up[k]; }
-    __syncthreads();
-    if (lane == 0 && live && ok) {
+    group_sync();
+    if (lane == 0 && ok) {
         spline_second(xs, ys, n, cps, cs);
         if (write_state) for (int i = 0; i < n; ++i) state.c_up[row + i] = cs[i];
         if (job.flags & kJobGreeks) {
@@ -792,13 +783,57 @@ __global__ __launch_bounds__(kBlock) void american_solve_kernel(const PdeJob* __
             gk.theta = (v_prev - v_cur) / dt * job.strike;
         }
     }
-    if (lane == 0 && live) {
+    if (lane == 0) {
         int32_t st = kStatusOk;
         if (!valid) st = params_valid(job) ? kStatusSolver : kStatusValidation;
         else if (!ok) st = kStatusSolver;
         if (st != kStatusOk) gk = Greeks{NAN, NAN, NAN, NAN};
         if (job.flags & kJobGreeks) greeks_out[job.out_index] = gk;
         status_out[job.out_index] = st;
+    }
+}
+
+// Solve kernel. Plain form: group g of block b takes job b*GPB + g. Work-queue
+// form (queue != nullptr): persistent groups pull job slots from *queue until
+// the count is exhausted. job_list (optional) maps slots to job indices, so a
+// size bin of a larger job array can be served in a chosen order.
+template <int K, int M, bool QUEUE>
+__global__ __launch_bounds__(kSolveBlock) void american_solve_kernel(const PdeJob* __restrict__ jobs, int n_jobs,
+                                                                     const int32_t* __restrict__ d_n_jobs,
+                                                                     const int32_t* __restrict__ job_list,
+                                                                     const int32_t* __restrict__ d_list_offset,
+                                                                     int32_t* __restrict__ queue,
+                                                                     Greeks* __restrict__ greeks_out,
+                                                                     int32_t* __restrict__ status_out,
+                                                                     StateOut state) {
+    constexpr int NMAX = K * M;
+    constexpr int GPB = kSolveBlock / K;
+    if (job_list && d_list_offset) job_list += *d_list_offset;
+    __shared__ double s_x[GPB][NMAX];
+    __shared__ double s_y[GPB][NMAX];
+    __shared__ double s_cp[GPB][NMAX];
+    __shared__ double s_c[GPB][NMAX];
+    if (d_n_jobs) n_jobs = min(n_jobs, *d_n_jobs);
+    const int slot = threadIdx.x / K;
+    const int lane = threadIdx.x % K;
+    if constexpr (!QUEUE) {
+        const int j = blockIdx.x * GPB + slot;
+        if (j >= n_jobs) return;  // the whole group leaves; synchronization is per wavefront
+        solve_one<K, M>(jobs, job_list ? job_list[j] : j, lane, s_x[slot], s_y[slot], s_cp[slot], s_c[slot],
+                        greeks_out, status_out, state);
+    } else {
+        for (;;) {
+            // Every lane takes part in the fetch (only the group's lane 0
+            // adds 1). A `if (lane == 0)` around the atomic would let jump
+            // threading route the other lanes from the epilogue's lane-0
+            // branch straight back to the shuffle, where they would read the
+            // inactive leader's register (zero) and repeat slot 0 forever.
+            const int t = atomicAdd(queue, lane == 0 ? 1 : 0);
+            const int j = __shfl(t, 0, K);
+            if (j >= n_jobs) break;
+            solve_one<K, M>(jobs, job_list ? job_list[j] : j, lane, s_x[slot], s_y[slot], s_cp[slot], s_c[slot],
+                            greeks_out, status_out, state);
+        }
     }
 }
 
@@ -932,6 +967,95 @@ __global__ __launch_bounds__(64) void plan_chains_kernel(const DeviceOptionRecor
     }
 }
 
+// ---------------------------------------------------------------------------
+// Size binning for the batched (uneven) form. Each option gets a job record,
+// its size bin (the solve-kernel variant that serves its grid size) and a cost
+// bucket; a counting sort lays each bin's jobs out in descending cost so the
+// work queue hands out the longest solves first (LPT order).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void fill_i32_kernel(int32_t* __restrict__ p, int n, int32_t v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void plan_binned_kernel(const DeviceOptionRecord* __restrict__ opts, int n,
+                                                             DevicePlan plan, BinTable bins, PdeJob* __restrict__ jobs,
+                                                             int32_t* __restrict__ keys, int32_t* __restrict__ hist) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DeviceOptionRecord o = opts[i];
+    PdeJob j;
+    j.maturity = o.maturity; j.volatility = o.volatility; j.rate = o.rate; j.dividend_yield = o.dividend_yield;
+    j.spot = o.spot; j.strike = o.strike; j.c_t = plan.accuracy.c_t; j.pad0 = 0.0;
+    j.option_type = o.option_type;
+    j.max_time_steps = plan.accuracy.max_time_steps;
+    j.min_time_steps = 0;
+    j.out_index = i;
+    j.flags = kJobGreeks;
+    const bool ok = o.spot > 0.0 && o.strike > 0.0 && o.maturity > 0.0 && o.volatility > 0.0 &&
+                    isfinite(o.spot) && isfinite(o.strike) && isfinite(o.maturity) && isfinite(o.volatility);
+    double nt_est;
+    if (plan.mode == 1) {
+        j.grid = plan.spec;
+        j.n_time = plan.n_time;
+        nt_est = plan.n_time;
+    } else {
+        j.grid = ok ? estimate_spatial(o.spot, o.strike, o.maturity, o.volatility, plan.accuracy) : plan.spec;
+        if (plan.mode == 2) {
+            j.n_time = plan.n_time;
+            nt_est = plan.n_time;
+        } else {
+            j.n_time = 0;
+            j.min_time_steps = plan.min_time_steps;
+            // step count from the raw (unrepaired) grid: only orders the work
+            double prev = raw_grid_point(j.grid, 0), dmin = 1e300;
+            for (int64_t k = 1; k < j.grid.n_points; ++k) {
+                const double x = raw_grid_point(j.grid, k);
+                dmin = fmin(dmin, x - prev);
+                prev = x;
+            }
+            nt_est = ceil(o.maturity / (plan.accuracy.c_t * fmax(dmin, 1e-300)));
+            nt_est = fmin(fmax(nt_est, static_cast<double>(plan.min_time_steps)),
+                          static_cast<double>(plan.accuracy.max_time_steps));
+        }
+    }
+    jobs[i] = j;
+    const int64_t ns = j.grid.n_points;
+    int b = bins.n_bins - 1;
+    for (int k = 0; k < bins.n_bins; ++k)
+        if (ns <= bins.max_n[k]) { b = k; break; }
+    const double cost = fmax(1.0, static_cast<double>(ns) * nt_est);
+    int bucket = static_cast<int>(log2(cost) * 2.0);  // half-octave buckets
+    bucket = bucket < 0 ? 0 : (bucket >= kCostBuckets ? kCostBuckets - 1 : bucket);
+    const int key = b * kCostBuckets + (kCostBuckets - 1 - bucket);
+    keys[i] = key;
+    atomicAdd(&hist[key], 1);
+}
+
+// Exclusive scan of the histogram (one block); per-bin start and count.
+__global__ __launch_bounds__(kBlock) void scan_bins_kernel(const int32_t* __restrict__ hist, int n_keys,
+                                                           int32_t* __restrict__ cursor, int32_t* __restrict__ bin_start,
+                                                           int32_t* __restrict__ bin_count, int n_bins) {
+    if (threadIdx.x != 0) return;
+    int32_t acc = 0;
+    for (int k = 0; k < n_keys; ++k) {
+        if (k % kCostBuckets == 0) bin_start[k / kCostBuckets] = acc;
+        cursor[k] = acc;
+        acc += hist[k];
+    }
+    for (int b = 0; b < n_bins; ++b) {
+        const int32_t end = (b + 1 < n_bins) ? bin_start[b + 1] : acc;
+        bin_count[b] = end - bin_start[b];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void scatter_bins_kernel(const int32_t* __restrict__ keys, int n,
+                                                              int32_t* __restrict__ cursor, int32_t* __restrict__ list) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    list[atomicAdd(&cursor[keys[i]], 1)] = i;
+}
+
 __global__ __launch_bounds__(kBlock) void plan_jobs_kernel(const DeviceOptionRecord* __restrict__ opts, int n,
                                                            DevicePlan plan, PdeJob* __restrict__ jobs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -970,11 +1094,26 @@ __global__ __launch_bounds__(kBlock) void plan_jobs_kernel(const DeviceOptionRec
 
 template <int K, int M>
 int launch_variant(const PdeJob* d_jobs, int n_jobs, const int32_t* d_n_jobs, Greeks* d_greeks, int32_t* d_status,
-                   const StateOut& state, hipStream_t stream) {
-    constexpr int GPB = kBlock / K;
-    const int blocks = (n_jobs + GPB - 1) / GPB;
-    hipLaunchKernelGGL((american_solve_kernel<K, M>), dim3(blocks), dim3(kBlock), 0, stream, d_jobs, n_jobs,
-                       d_n_jobs, d_greeks, d_status, state);
+                   const StateOut& state, hipStream_t stream, const int32_t* job_list, int32_t* queue,
+                   int max_blocks, const int32_t* d_list_offset) {
+    constexpr int GPB = kSolveBlock / K;
+    int blocks = (n_jobs + GPB - 1) / GPB;
+    if (queue) {
+        int per_cu = 0, dev = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, american_solve_kernel<K, M, true>, kSolveBlock, 0);
+        const int resident = std::max(1, per_cu) * std::max(1, cus);
+        blocks = std::min(blocks, resident);
+        if (max_blocks > 0) blocks = std::min(blocks, max_blocks);
+        if (blocks <= 0) return 0;
+        hipLaunchKernelGGL((american_solve_kernel<K, M, true>), dim3(blocks), dim3(kSolveBlock), 0, stream, d_jobs,
+                           n_jobs, d_n_jobs, job_list, d_list_offset, queue, d_greeks, d_status, state);
+    } else {
+        if (blocks <= 0) return 0;
+        hipLaunchKernelGGL((american_solve_kernel<K, M, false>), dim3(blocks), dim3(kSolveBlock), 0, stream, d_jobs,
+                           n_jobs, d_n_jobs, job_list, d_list_offset, queue, d_greeks, d_status, state);
+    }
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -1031,10 +1170,13 @@ SolveVariant pick_variant(int64_t n_space) {
 }
 
 int launch_american_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, Greeks* d_greeks,
-                          int32_t* d_status, const StateOut& state, hipStream_t stream, const int32_t* d_n_jobs) {
+                          int32_t* d_status, const StateOut& state, hipStream_t stream, const int32_t* d_n_jobs,
+                          const int32_t* job_list, int32_t* queue, int max_blocks, const int32_t* d_list_offset) {
     if (n_jobs <= 0) return 0;
 #define MANGO_LAUNCH(K, M) \
-    if (v.lanes == K && v.nodes == M) return launch_variant<K, M>(d_jobs, n_jobs, d_n_jobs, d_greeks, d_status, state, stream);
+    if (v.lanes == K && v.nodes == M) \
+        return launch_variant<K, M>(d_jobs, n_jobs, d_n_jobs, d_greeks, d_status, state, stream, job_list, queue, \
+                                    max_blocks, d_list_offset);
     MANGO_VARIANTS(MANGO_LAUNCH)
 #undef MANGO_LAUNCH
     return 2;
@@ -1045,6 +1187,26 @@ int launch_plan_jobs(const DeviceOptionRecord* d_opts, int n, const DevicePlan& 
     if (n <= 0) return 0;
     hipLaunchKernelGGL(plan_jobs_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d_opts, n, plan,
                        d_jobs);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_fill_i32(int32_t* p, int n, int32_t v, hipStream_t stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, p, n, v);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_plan_binned(const DeviceOptionRecord* d_opts, int n, const DevicePlan& plan, const BinTable& bins,
+                       PdeJob* d_jobs, int32_t* d_keys, int32_t* d_hist, int32_t* d_cursor, int32_t* d_bin_start,
+                       int32_t* d_bin_count, int32_t* d_list, hipStream_t stream) {
+    if (n <= 0) return 0;
+    const int n_keys = bins.n_bins * kCostBuckets;
+    hipLaunchKernelGGL(plan_binned_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d_opts, n, plan,
+                       bins, d_jobs, d_keys, d_hist);
+    hipLaunchKernelGGL(scan_bins_kernel, dim3(1), dim3(64), 0, stream, d_hist, n_keys, d_cursor, d_bin_start,
+                       d_bin_count, bins.n_bins);
+    hipLaunchKernelGGL(scatter_bins_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d_keys, n,
+                       d_cursor, d_list);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

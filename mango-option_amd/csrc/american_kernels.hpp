@@ -65,9 +65,14 @@ struct SolveVariant {
 
 // Host launchers (american_kernels.hip).
 // d_n_jobs (optional, device): the kernel processes min(n_jobs, *d_n_jobs) jobs.
+// job_list (optional, device): slot -> job index. queue (optional, device,
+// zeroed by the caller): persistent work-queue form with at most max_blocks
+// (0 = resident capacity) one-wavefront blocks.
 int launch_american_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs,
                           Greeks* d_greeks, int32_t* d_status, const StateOut& state,
-                          hipStream_t stream, const int32_t* d_n_jobs = nullptr);
+                          hipStream_t stream, const int32_t* d_n_jobs = nullptr,
+                          const int32_t* job_list = nullptr, int32_t* queue = nullptr, int max_blocks = 0,
+                          const int32_t* d_list_offset = nullptr);
 bool variant_supported(const SolveVariant& v);
 int list_variants(SolveVariant* out, int cap);
 int max_nodes_supported();
@@ -108,6 +113,22 @@ struct DeviceOptionRecord {  // layout of MangoDeviceOption
 
 int launch_plan_jobs(const DeviceOptionRecord* d_opts, int n, const DevicePlan& plan, PdeJob* d_jobs,
                      hipStream_t stream);
+
+// Size bins of the batched form: bin k serves grids with n_points <= max_n[k].
+constexpr int kMaxBins = 8;
+constexpr int kCostBuckets = 64;
+struct BinTable {
+    int32_t n_bins;
+    int32_t pad;
+    int64_t max_n[kMaxBins];
+};
+int launch_fill_i32(int32_t* p, int n, int32_t v, hipStream_t stream);
+// Job records, per-option (bin, cost) keys, counting sort: d_list holds the
+// job indices of bin b at [bin_start[b], bin_start[b] + bin_count[b]) in
+// descending cost. d_hist must be zeroed (bins * kCostBuckets ints).
+int launch_plan_binned(const DeviceOptionRecord* d_opts, int n, const DevicePlan& plan, const BinTable& bins,
+                       PdeJob* d_jobs, int32_t* d_keys, int32_t* d_hist, int32_t* d_cursor, int32_t* d_bin_start,
+                       int32_t* d_bin_count, int32_t* d_list, hipStream_t stream);
 // Segments [seg_offsets[s], seg_offsets[s+1]) each share one grid; emits one
 // job per distinct PDE (count in *d_job_count, which must start at 0) and one
 // ChainOption per option. d_rep is scratch of n ints.

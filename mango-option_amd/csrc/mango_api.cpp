@@ -488,20 +488,27 @@ bool g_timing = false;
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_events;
 }  // namespace
 
-int timed_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, Greeks* d_greeks, int32_t* d_status,
-                const StateOut& state, hipStream_t stream, const int32_t* d_n_jobs) {
+int timed_solve_ex(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, Greeks* d_greeks, int32_t* d_status,
+                   const StateOut& state, hipStream_t stream, const int32_t* d_n_jobs, const int32_t* job_list,
+                   int32_t* queue, const int32_t* d_list_offset) {
     hipEvent_t a = nullptr, b = nullptr;
     if (g_timing) {
         (void)hipEventCreate(&a);
         (void)hipEventCreate(&b);
         (void)hipEventRecord(a, stream);
     }
-    const int rc = launch_american_solve(v, d_jobs, n_jobs, d_greeks, d_status, state, stream, d_n_jobs);
+    const int rc = launch_american_solve(v, d_jobs, n_jobs, d_greeks, d_status, state, stream, d_n_jobs, job_list,
+                                         queue, 0, d_list_offset);
     if (g_timing) {
         (void)hipEventRecord(b, stream);
         g_events.push_back({a, b});
     }
     return rc;
+}
+
+int timed_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, Greeks* d_greeks, int32_t* d_status,
+                const StateOut& state, hipStream_t stream, const int32_t* d_n_jobs) {
+    return timed_solve_ex(v, d_jobs, n_jobs, d_greeks, d_status, state, stream, d_n_jobs, nullptr, nullptr, nullptr);
 }
 
 // Grow-only device scratch for the device-resident chain path.
@@ -587,6 +594,217 @@ MangoStatus device_chains(const MangoDeviceOption* d_opts, uint64_t n, const int
                                      d_status, stream) == 0;
     if (!ok) {
         set_err(err, MANGO_ERR_SOLVER, (std::string("HIP launch error: ") + hipGetErrorString(hipGetLastError())).c_str());
+        return MANGO_ERR_SOLVER;
+    }
+    return MANGO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: size bins + LPT order + persistent work queues, replayed from
+// a captured hipGraph.
+// ---------------------------------------------------------------------------
+struct QueueScratch {
+    size_t cap = 0;
+    PdeJob* jobs = nullptr;
+    int32_t* keys = nullptr;
+    int32_t* list = nullptr;
+    int32_t* ints = nullptr;  // hist | cursor (kMaxBins*kCostBuckets each) | bin_start | bin_count | queues
+    void release() {
+        for (void* p : {static_cast<void*>(jobs), static_cast<void*>(keys), static_cast<void*>(list),
+                        static_cast<void*>(ints)})
+            if (p) (void)hipFree(p);
+        *this = QueueScratch{};
+    }
+    static constexpr int kInts = 2 * kMaxBins * kCostBuckets + 3 * kMaxBins;
+    int32_t* hist() { return ints; }
+    int32_t* cursor() { return ints + kMaxBins * kCostBuckets; }
+    int32_t* bin_start() { return ints + 2 * kMaxBins * kCostBuckets; }
+    int32_t* bin_count() { return bin_start() + kMaxBins; }
+    int32_t* queues() { return bin_count() + kMaxBins; }
+    bool ensure(size_t n) {
+        if (n <= cap) return true;
+        release();
+        bool ok = hipMalloc(reinterpret_cast<void**>(&jobs), n * sizeof(PdeJob)) == hipSuccess;
+        ok = ok && hipMalloc(reinterpret_cast<void**>(&keys), n * sizeof(int32_t)) == hipSuccess;
+        ok = ok && hipMalloc(reinterpret_cast<void**>(&list), n * sizeof(int32_t)) == hipSuccess;
+        ok = ok && hipMalloc(reinterpret_cast<void**>(&ints), kInts * sizeof(int32_t)) == hipSuccess;
+        if (!ok) { release(); return false; }
+        cap = n;
+        return true;
+    }
+};
+QueueScratch g_queue;
+
+// A captured batch, replayable on any stream (the graph holds no stream).
+struct GraphEntry {
+    const void* opts;
+    uint64_t n;
+    const void* out;
+    const void* status;
+    MangoGridConfig cfg;
+    int32_t persistent;
+    int rules_version;
+    size_t scratch_cap;
+    hipGraphExec_t exec;
+};
+std::vector<GraphEntry> g_graphs;
+// One branch stream per size bin (fork/join with events) and a stream that
+// graph capture starts from: the caller's stream may be the legacy null
+// stream, which cannot be captured. All are created before any capture.
+std::vector<hipStream_t> g_aux_streams;
+std::vector<hipEvent_t> g_aux_events;
+hipStream_t g_capture_stream = nullptr;
+int g_rules_version = 0;
+
+bool ensure_aux_streams() {
+    if (!g_capture_stream && hipStreamCreateWithFlags(&g_capture_stream, hipStreamNonBlocking) != hipSuccess) {
+        g_capture_stream = nullptr;
+        return false;
+    }
+    while (g_aux_streams.size() < static_cast<size_t>(kMaxBins)) {
+        hipStream_t s;
+        hipEvent_t e1, e2;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return false;
+        if (hipEventCreateWithFlags(&e1, hipEventDisableTiming) != hipSuccess) return false;
+        if (hipEventCreateWithFlags(&e2, hipEventDisableTiming) != hipSuccess) return false;
+        g_aux_streams.push_back(s);
+        g_aux_events.push_back(e1);
+        g_aux_events.push_back(e2);
+    }
+    return true;
+}
+
+// Bins for the grid sizes a configuration can produce.
+BinTable make_bins(const MangoGridConfig& grid, std::vector<SolveVariant>& variants) {
+    int64_t lo, hi;
+    if (grid.mode == MANGO_GRID_MODE_EXPLICIT) {
+        lo = hi = static_cast<int64_t>(grid.spec.n_points);
+    } else {
+        lo = static_cast<int64_t>(grid.accuracy.min_spatial_points);
+        hi = static_cast<int64_t>(grid.accuracy.max_spatial_points);
+        if (lo % 2 == 0) ++lo;
+        if (hi % 2 == 0) ++hi;
+    }
+    lo = std::max<int64_t>(lo, 3);
+    hi = std::max(hi, lo);
+    BinTable t{};
+    variants.clear();
+    const int64_t cap = max_nodes_supported();
+    for (int64_t n = lo; n <= std::min(hi, cap) && t.n_bins < kMaxBins; ++n) {
+        const SolveVariant v = pick_variant(n);
+        if (t.n_bins > 0 && variants.back().lanes == v.lanes && variants.back().nodes == v.nodes) {
+            t.max_n[t.n_bins - 1] = n;
+        } else {
+            variants.push_back(v);
+            t.max_n[t.n_bins++] = n;
+        }
+    }
+    if (t.n_bins == 0) { variants.push_back(pick_variant(cap)); t.max_n[t.n_bins++] = cap; }
+    t.max_n[t.n_bins - 1] = std::max<int64_t>(t.max_n[t.n_bins - 1], hi);  // larger grids fail validation in-kernel
+    return t;
+}
+
+bool enqueue_binned(const MangoDeviceOption* d_opts, int n, const MangoGridConfig& grid, Greeks* d_out,
+                    int32_t* d_status, hipStream_t stream, bool persistent) {
+    DevicePlan plan{};
+    plan.mode = grid.mode;
+    plan.n_time = static_cast<int32_t>(grid.n_time);
+    plan.min_time_steps = static_cast<int64_t>(grid.min_time_steps);
+    plan.accuracy = to_accuracy(&grid.accuracy);
+    plan.spec = to_desc(grid.spec);
+    std::vector<SolveVariant> variants;
+    const BinTable bins = make_bins(grid, variants);
+    bool ok = launch_fill_i32(g_queue.ints, QueueScratch::kInts, 0, stream) == 0;
+    ok = ok && launch_plan_binned(reinterpret_cast<const DeviceOptionRecord*>(d_opts), n, plan, bins, g_queue.jobs,
+                                  g_queue.keys, g_queue.hist(), g_queue.cursor(), g_queue.bin_start(),
+                                  g_queue.bin_count(), g_queue.list, stream) == 0;
+    // one branch per bin so bins run side by side
+    if (g_aux_streams.size() < static_cast<size_t>(bins.n_bins)) return false;
+    hipEvent_t fork = g_aux_events[0];
+    ok = ok && hipEventRecord(fork, stream) == hipSuccess;
+    for (int b = 0; b < bins.n_bins && ok; ++b) {
+        hipStream_t s = g_aux_streams[b];
+        ok = hipStreamWaitEvent(s, fork, 0) == hipSuccess;
+        ok = ok && timed_solve_ex(variants[b], g_queue.jobs, n, d_out, d_status, StateOut{}, s, g_queue.bin_count() + b,
+                                  g_queue.list, persistent ? g_queue.queues() + b : nullptr,
+                                  g_queue.bin_start() + b) == 0;
+        ok = ok && hipEventRecord(g_aux_events[2 * b + 1], s) == hipSuccess;
+    }
+    for (int b = 0; b < bins.n_bins && ok; ++b) ok = hipStreamWaitEvent(stream, g_aux_events[2 * b + 1], 0) == hipSuccess;
+    return ok;
+}
+
+MangoStatus device_queue(const MangoDeviceOption* d_opts, uint64_t n, const MangoGridConfig* grid,
+                         MangoAmericanGreeks* d_out, int32_t* d_status, hipStream_t stream,
+                         const MangoQueueConfig* qcfg, MangoError* err) {
+    if (n == 0) return MANGO_OK;
+    if (!d_opts || !d_out || !d_status || !grid) { set_err(err, MANGO_ERR_VALIDATION, "null argument"); return MANGO_ERR_VALIDATION; }
+    if (n > static_cast<uint64_t>(std::numeric_limits<int32_t>::max() / 2)) { set_err(err, MANGO_ERR_VALIDATION, "batch too large"); return MANGO_ERR_VALIDATION; }
+    if (grid->mode == MANGO_GRID_MODE_EXPLICIT && (grid->spec.n_points < 3 || grid->n_time == 0)) {
+        set_err(err, MANGO_ERR_VALIDATION, "explicit grid needs n_points >= 3 and n_time > 0");
+        return MANGO_ERR_VALIDATION;
+    }
+    if (grid->mode == MANGO_GRID_MODE_ACCURACY_FIXED_TIME && grid->n_time == 0) {
+        set_err(err, MANGO_ERR_VALIDATION, "n_time must be > 0");
+        return MANGO_ERR_VALIDATION;
+    }
+    if (grid->mode != MANGO_GRID_MODE_EXPLICIT && grid->mode != MANGO_GRID_MODE_ACCURACY &&
+        grid->mode != MANGO_GRID_MODE_ACCURACY_FIXED_TIME) {
+        set_err(err, MANGO_ERR_VALIDATION, "unknown grid mode");
+        return MANGO_ERR_VALIDATION;
+    }
+    if (!device_ready(err)) return MANGO_ERR_SOLVER;
+    const bool persistent = qcfg ? qcfg->persistent != 0 : true;
+    const bool use_graph = qcfg ? qcfg->use_graph != 0 : true;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n > g_queue.cap) {
+        for (auto& e : g_graphs) (void)hipGraphExecDestroy(e.exec);
+        g_graphs.clear();
+        if (!g_queue.ensure(n)) { set_err(err, MANGO_ERR_SOLVER, "hipMalloc failed"); return MANGO_ERR_SOLVER; }
+    }
+    const int ni = static_cast<int>(n);
+    auto* out = reinterpret_cast<Greeks*>(d_out);
+    if (!ensure_aux_streams()) {
+        set_err(err, MANGO_ERR_SOLVER, "hipStreamCreate failed");
+        return MANGO_ERR_SOLVER;
+    }
+    if (!use_graph || g_timing) {
+        if (!enqueue_binned(d_opts, ni, *grid, out, d_status, stream, persistent)) {
+            set_err(err, MANGO_ERR_SOLVER, (std::string("HIP launch error: ") + hipGetErrorString(hipGetLastError())).c_str());
+            return MANGO_ERR_SOLVER;
+        }
+        return MANGO_OK;
+    }
+    for (auto& e : g_graphs) {
+        if (e.opts == d_opts && e.n == n && e.out == d_out && e.status == d_status &&
+            e.persistent == persistent && e.rules_version == g_rules_version && e.scratch_cap == g_queue.cap &&
+            std::memcmp(&e.cfg, grid, sizeof(MangoGridConfig)) == 0) {
+            if (hipGraphLaunch(e.exec, stream) != hipSuccess) {
+                set_err(err, MANGO_ERR_SOLVER, "hipGraphLaunch failed");
+                return MANGO_ERR_SOLVER;
+            }
+            return MANGO_OK;
+        }
+    }
+    // Capture from the library's own stream; the graph is then launched on
+    // the caller's stream, which orders it after the caller's earlier work.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cs = g_capture_stream;
+    bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    const bool enq = ok && enqueue_binned(d_opts, ni, *grid, out, d_status, cs, persistent);
+    ok = hipStreamEndCapture(cs, &graph) == hipSuccess && ok && enq;
+    ok = ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!ok) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        set_err(err, MANGO_ERR_SOLVER, (std::string("graph capture failed: ") + hipGetErrorString(hipGetLastError())).c_str());
+        return MANGO_ERR_SOLVER;
+    }
+    GraphEntry e{d_opts, n, d_out, d_status, *grid, persistent ? 1 : 0, g_rules_version, g_queue.cap, exec};
+    g_graphs.push_back(e);
+    if (hipGraphLaunch(exec, stream) != hipSuccess) {
+        set_err(err, MANGO_ERR_SOLVER, "hipGraphLaunch failed");
         return MANGO_ERR_SOLVER;
     }
     return MANGO_OK;
@@ -991,6 +1209,7 @@ MangoStatus mango_tuning_set(const MangoVariantRule* rules, uint64_t n_rules, Ma
     }
     std::lock_guard<std::mutex> lk(g_mu);
     set_variant_rules(mx.data(), vs.data(), static_cast<int>(mx.size()));
+    ++g_rules_version;
     return MANGO_OK;
 }
 
@@ -1000,6 +1219,13 @@ uint64_t mango_tuning_variants(MangoVariantRule* out, uint64_t cap) {
     for (int i = 0; i < c && static_cast<uint64_t>(i) < cap; ++i)
         out[i] = MangoVariantRule{static_cast<uint64_t>(v[i].lanes) * v[i].nodes, v[i].lanes, v[i].nodes};
     return static_cast<uint64_t>(c);
+}
+
+MangoStatus mango_price_american_queue_device(const MangoDeviceOption* d_opts, uint64_t n,
+                                              const MangoGridConfig* grid, MangoAmericanGreeks* d_out,
+                                              int32_t* d_status, void* stream, const MangoQueueConfig* queue_cfg,
+                                              MangoError* err) {
+    return device_queue(d_opts, n, grid, d_out, d_status, static_cast<hipStream_t>(stream), queue_cfg, err);
 }
 
 void mango_profile_enable(int32_t enable) {

@@ -135,7 +135,7 @@ __host__ __device__ inline GridDesc estimate_spatial(double spot, double strike,
 }
 
 // Whether generate() runs enforce_monotonicity for this spec.
-__host__ __device__ inline bool needs_monotonicity(const GridDesc& g) {
+__host__ __device__ __attribute__((always_inline)) inline bool needs_monotonicity(const GridDesc& g) {
     if (g.type != kMultiSinh) return false;
     if (g.n_clusters != 1) return true;
     const double eta_center = (g.cl_center[0] - g.x_min) / (g.x_max - g.x_min);
@@ -143,7 +143,7 @@ __host__ __device__ inline bool needs_monotonicity(const GridDesc& g) {
 }
 
 // Point i of GridSpec::generate before enforce_monotonicity.
-__host__ __device__ inline double raw_grid_point(const GridDesc& g, int64_t i) {
+__host__ __device__ __attribute__((always_inline)) inline double raw_grid_point(const GridDesc& g, int64_t i) {
     const int64_t n = g.n_points;
     switch (g.type) {
         case kUniform: {

@@ -107,6 +107,10 @@ class MangoVariantRule(C.Structure):
     _fields_ = [("max_n", C.c_uint64), ("lanes", C.c_int32), ("nodes_per_lane", C.c_int32)]
 
 
+class MangoQueueConfig(C.Structure):
+    _fields_ = [("persistent", C.c_int32), ("use_graph", C.c_int32)]
+
+
 class MangoDeviceOption(C.Structure):
     _fields_ = [
         ("spot", C.c_double),
@@ -166,6 +170,9 @@ SIGNATURES = {
     "mango_price_american_chains_device": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
                                                        _P(MangoGridConfig), C.c_void_p, C.c_void_p, C.c_void_p,
                                                        _P(MangoError)]),
+    "mango_price_american_queue_device": (C.c_int32, [C.c_void_p, C.c_uint64, _P(MangoGridConfig), C.c_void_p,
+                                                      C.c_void_p, C.c_void_p, _P(MangoQueueConfig),
+                                                      _P(MangoError)]),
     "mango_tuning_set": (C.c_int32, [_P(MangoVariantRule), C.c_uint64, _P(MangoError)]),
     "mango_tuning_variants": (C.c_uint64, [_P(MangoVariantRule), C.c_uint64]),
     "mango_profile_enable": (None, [C.c_int32]),
@@ -181,6 +188,13 @@ def load_library(path: str | None = None) -> C.CDLL:
     if _lib is not None and path is None:
         return _lib
     p = path or LIB_PATH
+    # PyTorch ships its own HIP runtime. Load it first when it is installed, so
+    # that this library binds to the same runtime instead of bringing a second
+    # one into the process (two runtimes cannot share the device).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(p):
         raise RuntimeError(
             f"{LIB_NAME} is not built ({p}); run `make -C mango-option_amd` "

@@ -1,0 +1,157 @@
+# SPDX-License-Identifier: MIT
+"""GPU tests of the device-resident entry points: plain device batch, the
+batched form (size bins, LPT order, persistent work queues, hipGraph replay)
+and device chains, against the oracle (CPU restatement of the reference)."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import mango_option as M
+from mango_option import _abi, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+PRICE_ABS = 1e-9
+GREEK_REL = 1e-7
+GREEK_FLOOR = 1e-10
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _check(g, s, gr, sr, label):
+    assert np.array_equal(s, sr), f"{label}: status"
+    ok = sr == 0
+    assert np.max(np.abs(g[ok, 0] - gr[ok, 0]), initial=0.0) <= PRICE_ABS, label
+    for j in (1, 2, 3):
+        d = np.abs(g[ok, j] - gr[ok, j])
+        assert (d <= GREEK_REL * np.abs(gr[ok, j]) + GREEK_FLOOR).all(), (label, j, d.max())
+
+
+class Dev:
+    def __init__(self, opts):
+        import torch
+        self.torch = torch
+        self.n = len(opts)
+        self.d_opts = torch.from_numpy(opts.records().view(np.uint8)).cuda()
+        self.d_out = torch.full((self.n, 4), float("nan"), dtype=torch.float64, device="cuda")
+        self.d_st = torch.full((self.n,), -1, dtype=torch.int32, device="cuda")
+        self.stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def result(self):
+        self.torch.cuda.synchronize()
+        return self.d_out.cpu().numpy().copy(), self.d_st.cpu().numpy().copy()
+
+
+def _queue(dev, cfg, persistent=1, use_graph=1):
+    lib = _abi.load_library()
+    err = _abi.MangoError()
+    q = _abi.MangoQueueConfig(persistent, use_graph)
+    rc = lib.mango_price_american_queue_device(dev.d_opts.data_ptr(), dev.n, C.byref(cfg), dev.d_out.data_ptr(),
+                                               dev.d_st.data_ptr(), dev.stream, C.byref(q), C.byref(err))
+    assert rc == 0, err.message.decode()
+    return dev.result()
+
+
+def _plain(dev, cfg):
+    lib = _abi.load_library()
+    err = _abi.MangoError()
+    rc = lib.mango_price_american_batch_device(dev.d_opts.data_ptr(), dev.n, C.byref(cfg), dev.d_out.data_ptr(),
+                                               dev.d_st.data_ptr(), dev.stream, C.byref(err))
+    assert rc == 0, err.message.decode()
+    return dev.result()
+
+
+def _uneven_cfg():
+    acc = M.GridAccuracyParams(tol=1e-3, min_spatial_points=51, max_spatial_points=401, max_time_steps=2000)
+    return M.grid_config(_abi.MANGO_GRID_MODE_ACCURACY, acc, min_time_steps=100)
+
+
+def test_device_plain_matches_oracle():
+    _say("test_device_plain_matches_oracle: 256 puts 101x498")
+    opts = W.c1_puts(256, seed=3)
+    acc = M.GridAccuracyParams(min_spatial_points=101, max_spatial_points=101)
+    cfg = M.grid_config(_abi.MANGO_GRID_MODE_ACCURACY_FIXED_TIME, acc, n_time=498)
+    g, s = _plain(Dev(opts), cfg)
+    gr, sr, _, _ = O.price_batch(opts, False, cfg, use_normalized=False, n_threads=8)
+    _check(g, s, gr, sr, "device-plain")
+
+
+@pytest.mark.parametrize("persistent", [1, 0])
+def test_queue_uneven_matches_oracle(persistent):
+    _say(f"test_queue_uneven_matches_oracle[persistent={persistent}]: 512 uneven options 51-401 x 100-2000")
+    opts = W.c3_uneven(512, seed=2)
+    cfg = _uneven_cfg()
+    g, s = _queue(Dev(opts), cfg, persistent=persistent, use_graph=0)
+    gr, sr, _, _ = O.price_batch(opts, False, cfg, use_normalized=False, n_threads=8)
+    _check(g, s, gr, sr, "queue-uneven")
+
+
+def test_queue_graph_replays_match_eager():
+    # every replay, with eager work in between, must equal the eager result
+    _say("test_queue_graph_replays_match_eager: 1024 uneven options, 3 replays")
+    opts = W.c3_uneven(1024, seed=4)
+    cfg = _uneven_cfg()
+    dev = Dev(opts)
+    g0, s0 = _queue(dev, cfg, use_graph=0)
+    for rep in range(3):
+        dev.d_out.fill_(float("nan"))
+        dev.d_st.fill_(-1)
+        g1, s1 = _queue(dev, cfg, use_graph=1)
+        assert np.array_equal(s1, s0)
+        assert np.array_equal(g1, g0, equal_nan=True), f"replay {rep}"
+        _plain(Dev(W.c1_puts(64, seed=rep)), cfg)  # unrelated eager work between replays
+
+
+def test_queue_fixed_grid_matches_plain_variant():
+    _say("test_queue_fixed_grid_matches_plain_variant: 2048 puts 101x498")
+    opts = W.c1_puts(2048, seed=6)
+    acc = M.GridAccuracyParams(min_spatial_points=101, max_spatial_points=101)
+    cfg = M.grid_config(_abi.MANGO_GRID_MODE_ACCURACY_FIXED_TIME, acc, n_time=498)
+    gq, sq = _queue(Dev(opts), cfg)
+    gp, sp = _plain(Dev(opts), cfg)
+    # same variant for a single size bin: identical arithmetic
+    assert np.array_equal(sq, sp) and np.array_equal(gq, gp)
+
+
+def test_queue_invalid_slots():
+    _say("test_queue_invalid_slots")
+    opts = W.c3_uneven(64, seed=8)
+    opts.volatility[5] = 0.0
+    opts.maturity[9] = -1.0
+    g, s = _queue(Dev(opts), _uneven_cfg(), use_graph=0)
+    assert s[5] == _abi.MANGO_ERR_VALIDATION and s[9] == _abi.MANGO_ERR_VALIDATION
+    assert np.isnan(g[5]).all() and np.isnan(g[9]).all()
+    assert (np.delete(s, [5, 9]) == 0).all()
+
+
+def test_device_chains_match_oracle():
+    _say("test_device_chains_match_oracle: 8 chains x 64 strikes")
+    opts, chain = W.c2_chains(8, 64, seed=1)
+    torch = pytest.importorskip("torch")
+    seg = np.arange(0, len(opts) + 1, 64, dtype=np.int64)
+    dev = Dev(opts)
+    d_seg = torch.from_numpy(seg).cuda()
+    acc = M.GridAccuracyParams(min_spatial_points=101, max_spatial_points=101)
+    cfg = M.grid_config(_abi.MANGO_GRID_MODE_ACCURACY_FIXED_TIME, acc, n_time=498)
+    err = _abi.MangoError()
+    rc = _abi.load_library().mango_price_american_chains_device(
+        dev.d_opts.data_ptr(), dev.n, d_seg.data_ptr(), len(seg) - 1, C.byref(cfg), dev.d_out.data_ptr(),
+        dev.d_st.data_ptr(), dev.stream, C.byref(err))
+    assert rc == 0, err.message.decode()
+    g, s = dev.result()
+    for c in range(8):
+        idx = np.arange(seg[c], seg[c + 1])
+        sub = opts.take(idx)
+        arr, n = O.c_params(sub)
+        spec, _ = O.estimate_batch_pde_grid(arr, n, acc.to_c())
+        ecfg = M.grid_config(_abi.MANGO_GRID_MODE_EXPLICIT, spec=M.GridSpec.from_c(spec), n_time=498)
+        for typ in (0, 1):
+            sel = np.where(sub.option_type == typ)[0]
+            gr, sr, _, _ = O.price_batch(sub.take(sel), True, ecfg, n_threads=8)
+            _check(g[idx[sel]], s[idx[sel]], gr, sr, f"chain{c}/{typ}")
