@@ -1,17 +1,23 @@
 # SPDX-License-Identifier: MIT
 """Benchmark: batched American-option PDE solve on one MI355X.
 
-Workload (the configuration the headline metric is quoted on, 101x498 FP64):
-shared-grid option chains, 64 chains x 64 strikes = 4096 options, per chain
-S=100, T~U[0.05,2], sigma~U[0.1,0.6], r~U[0,0.08], q~U[0,0.04], strikes
-70..130, puts/calls alternating, one 101-point grid per chain (spatial union
-of estimate_batch_pde_grid with 101 points) and 498 TR-BDF2 steps. One step =
-one call of mango_price_american_chains_device over the 4096 options with
-inputs already in HBM.
+Headline workload (101x498 FP64): shared-grid option chains, 64 chains x 64
+strikes = 4096 options, per chain S=100, T~U[0.05,2], sigma~U[0.1,0.6],
+r~U[0,0.08], q~U[0,0.04], strikes 70..130, puts/calls alternating, one
+101-point grid per chain (spatial union of estimate_batch_pde_grid with 101
+points) and 498 TR-BDF2 steps. One step = one call of
+mango_price_american_chains_device over the 4096 options with inputs already
+in HBM.
 
-Also measured: the plain form (one PDE per option, per-option 101-point
-multi-sinh grid x 498 steps) on 65,536 options of the plain-configuration
-distribution, and the oracle's OpenMP batch on the host cores (CPU baseline).
+Also measured in the same run:
+- plain form: one PDE per option, per-option 101-point multi-sinh grid x 498
+  steps, 65,536 puts of the plain-configuration distribution
+  (mango_price_american_batch_device);
+- batched form: 65,536 options of uneven grid sizes (51-401 points x 100-2000
+  steps) through the size-binned persistent work queues replayed from a
+  hipGraph (mango_price_american_queue_device);
+- cpu_baseline: the oracle (CPU restatement of the reference's OpenMP batch
+  path) on the host cores, which also checks a sample of the GPU results.
 """
 from __future__ import annotations
 
@@ -19,6 +25,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -29,17 +36,26 @@ sys.path.insert(0, os.path.join(ROOT, "mango-option_amd"))
 
 PEAK_FP64_TFLOPS = 78.6
 N_SPACE, N_TIME = 101, 498
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "american_solve_kernel_hbm_traffic.md")
 
 
-def flops_per_option(n: int, nt: int) -> float:
+def flops_per_option(n, nt):
     """Algorithmic FP64 work of one PDE solve: (n-2)[51(N-1) + 28 + 67]."""
-    return (n - 2) * (51 * (nt - 1) + 95)
+    return (np.asarray(n, dtype=np.float64) - 2) * (51 * (np.asarray(nt, dtype=np.float64) - 1) + 95)
 
 
 def chain_grid_config(M):
     from mango_option import _abi
     acc = M.GridAccuracyParams(min_spatial_points=N_SPACE, max_spatial_points=N_SPACE)
     return M.grid_config(_abi.MANGO_GRID_MODE_ACCURACY_FIXED_TIME, acc, n_time=N_TIME)
+
+
+def uneven_grid_config(M):
+    """Per-option auto grids: tol 1e-3, 51-401 points, at most 2000 and at
+    least 100 steps."""
+    from mango_option import _abi
+    acc = M.GridAccuracyParams(tol=1e-3, min_spatial_points=51, max_spatial_points=401, max_time_steps=2000)
+    return M.grid_config(_abi.MANGO_GRID_MODE_ACCURACY, acc, min_time_steps=100)
 
 
 def count_distinct_pdes(opts, seg):
@@ -53,154 +69,188 @@ def count_distinct_pdes(opts, seg):
     return n
 
 
-def run_gpu(args):
-    import torch
-    import mango_option as M
-    from mango_option import _abi, workloads as W
+def traffic_from_summary(workload: str):
+    """HBM bytes per launch of the solve kernel for `workload`, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-kernel scaled);
+    None when no summary covers it."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            text = f.read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\[" + re.escape(workload) + r"\]\s*=\s*([0-9.eE+]+)", text, re.M)
+    return float(m.group(1)) if m else None
 
-    lib = _abi.load_library()
-    if args.variant:
-        k, m = (int(v) for v in args.variant.split(","))
-        rule = (_abi.MangoVariantRule * 1)(_abi.MangoVariantRule(N_SPACE, k, m))
-        if lib.mango_tuning_set(rule, 1, None) != 0:
-            raise SystemExit(f"variant {args.variant} is not compiled")
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream(dev)
-    sp = C.c_void_p(stream.cuda_stream)
-    err = _abi.MangoError()
 
-    # ---- chain workload ----
-    opts, chain = W.c2_chains(64, 64, seed=1)
+class Gpu:
+    def __init__(self, variant: str):
+        import torch
+        import mango_option as M
+        from mango_option import _abi, workloads as W
+        self.torch, self.M, self.abi, self.W = torch, M, _abi, W
+        self.lib = _abi.load_library()
+        if variant:
+            k, m = (int(v) for v in variant.split(","))
+            rule = (_abi.MangoVariantRule * 1)(_abi.MangoVariantRule(N_SPACE, k, m))
+            if self.lib.mango_tuning_set(rule, 1, None) != 0:
+                raise SystemExit(f"variant {variant} is not compiled")
+        self.dev = torch.device("cuda:0")
+        self.sp = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+        self.err = _abi.MangoError()
+
+    def upload(self, batch):
+        t = self.torch
+        n = len(batch)
+        return (t.from_numpy(batch.records().view(np.uint8)).to(self.dev),
+                t.empty((n, 4), dtype=t.float64, device=self.dev), t.empty(n, dtype=t.int32, device=self.dev))
+
+    def check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.err.message.decode())
+
+    def timed(self, step, steps, warmup, kernel_timing=True):
+        """Wall time of `steps` calls bracketed by device syncs, and the
+        average solve-kernel duration from HIP events on the launch stream."""
+        torch, lib = self.torch, self.lib
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        if kernel_timing:
+            lib.mango_profile_enable(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        kms, kl = C.c_double(), C.c_uint64()
+        if kernel_timing:
+            lib.mango_profile_read(C.byref(kms), C.byref(kl))
+            lib.mango_profile_enable(0)
+        return el, (kms.value / max(1, kl.value) if kernel_timing else None), int(kl.value)
+
+
+def run_chains(g: Gpu, args):
+    opts, _ = g.W.c2_chains(64, 64, seed=1)
     n = len(opts)
     seg = np.arange(0, n + 1, 64, dtype=np.int64)
-    n_pde = count_distinct_pdes(opts, seg)
-    d_opts = torch.from_numpy(opts.records().view(np.uint8)).to(dev)
-    d_seg = torch.from_numpy(seg).to(dev)
-    d_out = torch.empty((n, 4), dtype=torch.float64, device=dev)
-    d_st = torch.empty(n, dtype=torch.int32, device=dev)
-    cfg = chain_grid_config(M)
+    d_opts, d_out, d_st = g.upload(opts)
+    d_seg = g.torch.from_numpy(seg).to(g.dev)
+    cfg = chain_grid_config(g.M)
 
-    def chain_step():
-        rc = lib.mango_price_american_chains_device(d_opts.data_ptr(), n, d_seg.data_ptr(), len(seg) - 1,
-                                                     C.byref(cfg), d_out.data_ptr(), d_st.data_ptr(), sp,
-                                                     C.byref(err))
-        if rc != 0:
-            raise RuntimeError(err.message.decode())
+    def step():
+        g.check(g.lib.mango_price_american_chains_device(d_opts.data_ptr(), n, d_seg.data_ptr(), len(seg) - 1,
+                                                         C.byref(cfg), d_out.data_ptr(), d_st.data_ptr(), g.sp,
+                                                         C.byref(g.err)))
 
-    for _ in range(args.warmup):
-        chain_step()
-    torch.cuda.synchronize()
-    lib.mango_profile_enable(1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        chain_step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    kms = C.c_double()
-    kl = C.c_uint64()
-    lib.mango_profile_read(C.byref(kms), C.byref(kl))
-    lib.mango_profile_enable(0)
-    status = d_st.cpu().numpy()
-    greeks = d_out.cpu().numpy()
-    if (status != 0).any():
-        raise RuntimeError(f"{int((status != 0).sum())} chain options failed")
-    chain = {
-        "ms_per_step": 1e3 * elapsed / args.steps,
-        "options_per_s": n * args.steps / elapsed,
-        "kernel_ms": kms.value / max(1, kl.value),
-        "launches": int(kl.value),
-        "pde_solves": n_pde,
-        "greeks": greeks,
-    }
-
-    # ---- plain form: one PDE per option, per-option grids ----
-    plain = None
-    if args.plain_options > 0:
-        base = W.c1_puts(min(args.plain_options, 4096), seed=0)
-        popts = W.replicate(base, args.plain_options)
-        pn = len(popts)
-        d_p = torch.from_numpy(popts.records().view(np.uint8)).to(dev)
-        d_po = torch.empty((pn, 4), dtype=torch.float64, device=dev)
-        d_ps = torch.empty(pn, dtype=torch.int32, device=dev)
-
-        def plain_step():
-            rc = lib.mango_price_american_batch_device(d_p.data_ptr(), pn, C.byref(cfg), d_po.data_ptr(),
-                                                        d_ps.data_ptr(), sp, C.byref(err))
-            if rc != 0:
-                raise RuntimeError(err.message.decode())
-
-        plain_step()
-        torch.cuda.synchronize()
-        lib.mango_profile_enable(1)
-        t0 = time.perf_counter()
-        for _ in range(args.plain_steps):
-            plain_step()
-        torch.cuda.synchronize()
-        pel = time.perf_counter() - t0
-        lib.mango_profile_read(C.byref(kms), C.byref(kl))
-        lib.mango_profile_enable(0)
-        ps = d_ps.cpu().numpy()
-        if (ps != 0).any():
-            raise RuntimeError(f"{int((ps != 0).sum())} plain options failed")
-        plain = {"options": pn, "ms_per_step": 1e3 * pel / args.plain_steps,
-                 "options_per_s": pn * args.plain_steps / pel, "kernel_ms": kms.value / max(1, kl.value),
-                 "greeks_head": d_po[:64].cpu().numpy()}
-    return opts, seg, chain, plain
+    el, kms, launches = g.timed(step, args.steps, args.warmup)
+    st = d_st.cpu().numpy()
+    if (st != 0).any():
+        raise RuntimeError(f"{int((st != 0).sum())} chain options failed")
+    return {"opts": opts, "seg": seg, "ms_per_step": 1e3 * el / args.steps, "options_per_s": n * args.steps / el,
+            "kernel_ms": kms, "launches": launches, "pde_solves": count_distinct_pdes(opts, seg),
+            "greeks": d_out.cpu().numpy()}
 
 
-def parity_check(opts, seg, greeks, n_chains=2):
-    """First chains against the oracle with the reference's routing."""
-    import mango_option as M
-    from mango_option import _abi
-    import pyoracle as O
-    cfg_acc = M.GridAccuracyParams(min_spatial_points=N_SPACE, max_spatial_points=N_SPACE).to_c()
-    worst = 0.0
-    for c in range(n_chains):
-        idx = np.arange(seg[c], seg[c + 1])
-        sub = opts.take(idx)
-        arr, n = O.c_params(sub)
-        spec, _ = O.estimate_batch_pde_grid(arr, n, cfg_acc)
-        cfg = M.grid_config(_abi.MANGO_GRID_MODE_EXPLICIT, spec=M.GridSpec.from_c(spec), n_time=N_TIME)
-        for typ in (0, 1):
-            sel = np.where(sub.option_type == typ)[0]
-            g, s, _, _ = O.price_batch(sub.take(sel), True, cfg, n_threads=os.cpu_count() or 1)
-            worst = max(worst, float(np.max(np.abs(g[:, 0] - greeks[idx[sel], 0]))))
-    return worst
+def run_plain(g: Gpu, args):
+    base = g.W.c1_puts(min(args.plain_options, 4096), seed=0)
+    popts = g.W.replicate(base, args.plain_options)
+    n = len(popts)
+    d_opts, d_out, d_st = g.upload(popts)
+    cfg = chain_grid_config(g.M)
+
+    def step():
+        g.check(g.lib.mango_price_american_batch_device(d_opts.data_ptr(), n, C.byref(cfg), d_out.data_ptr(),
+                                                        d_st.data_ptr(), g.sp, C.byref(g.err)))
+
+    el, kms, _ = g.timed(step, args.plain_steps, 1)
+    st = d_st.cpu().numpy()
+    if (st != 0).any():
+        raise RuntimeError(f"{int((st != 0).sum())} plain options failed")
+    ach = n * float(flops_per_option(N_SPACE, N_TIME)) / (kms * 1e-3) / 1e12
+    return {"workload": f"{n} puts of the plain configuration distribution, per-option 101-point grids x 498 "
+                        f"steps, one PDE per option",
+            "options": n, "options_per_s": n * args.plain_steps / el, "ms_per_step": 1e3 * el / args.plain_steps,
+            "kernel_ms": kms, "roofline": {"achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                                           "frac": ach / PEAK_FP64_TFLOPS},
+            "_batch": popts.take(np.arange(64)), "_greeks": d_out[:64].cpu().numpy()}
 
 
-def cpu_baseline(opts, seg, seconds: float):
+def run_batched(g: Gpu, args):
+    opts = g.W.c3_uneven(args.batched_options, seed=2)
+    n = len(opts)
+    cfg = uneven_grid_config(g.M)
+    ns, nt = g.M.plan_grid_sizes(opts, cfg)
+    total_flops = float(np.sum(flops_per_option(ns, nt)))
+    d_opts, d_out, d_st = g.upload(opts)
+    qcfg = g.abi.MangoQueueConfig(1, 1)
+
+    def step():
+        g.check(g.lib.mango_price_american_queue_device(d_opts.data_ptr(), n, C.byref(cfg), d_out.data_ptr(),
+                                                        d_st.data_ptr(), g.sp, C.byref(qcfg), C.byref(g.err)))
+
+    el, _, _ = g.timed(step, args.batched_steps, 1, kernel_timing=False)
+    st = d_st.cpu().numpy()
+    if (st != 0).any():
+        raise RuntimeError(f"{int((st != 0).sum())} batched options failed")
+    ms = 1e3 * el / args.batched_steps
+    ach = total_flops / (ms * 1e-3) / 1e12
+    return {"workload": f"{n} options of uneven size (per-option auto grids {int(ns.min())}-{int(ns.max())} points "
+                        f"x {int(nt.min())}-{int(nt.max())} steps), size-binned persistent work queues replayed "
+                        f"from a hipGraph",
+            "options": n, "options_per_s": n / (ms * 1e-3), "ms_per_step": ms,
+            "mean_points": float(ns.mean()), "mean_steps": float(nt.mean()), "flops_per_step": total_flops,
+            "roofline": {"achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS,
+                         "note": "whole step (planning + all bins) wall time"},
+            "_batch": opts.take(np.arange(args.batched_check)), "_greeks": d_out[:args.batched_check].cpu().numpy(),
+            "_cfg": cfg}
+
+
+def cpu_baseline(chains, batched, seconds: float):
     """The oracle (CPU restatement of the reference) pricing whole chains the
     reference's way: one solve_batch(group, use_shared_grid=true, grid) per
     (chain, type) with the chain's union grid; OpenMP across the regular
-    route's options. Chains are timed until `seconds` elapse."""
+    route's options. Chains are timed until `seconds` elapse. The oracle's
+    results for the timed chains (and for the first batched options) are
+    compared with the GPU's."""
     import mango_option as M
     from mango_option import _abi
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     threads = min(os.cpu_count() or 1, 16)
+    opts, seg, greeks = chains["opts"], chains["seg"], chains["greeks"]
     acc = M.GridAccuracyParams(min_spatial_points=N_SPACE, max_spatial_points=N_SPACE).to_c()
-    done = 0
-    solves = 0
+    n_chains = len(seg) - 1
+    done = solves = c = 0
+    worst = 0.0
     t0 = time.perf_counter()
-    c = 0
-    while c < len(seg) - 1:
-        idx = np.arange(seg[c], seg[c + 1])
+    # whole passes over the chains until `seconds` elapse (at least one pass)
+    while c < n_chains or time.perf_counter() - t0 < seconds:
+        k = c % n_chains
+        idx = np.arange(seg[k], seg[k + 1])
         sub = opts.take(idx)
         arr, n = O.c_params(sub)
         spec, _ = O.estimate_batch_pde_grid(arr, n, acc)
         cfg = M.grid_config(_abi.MANGO_GRID_MODE_EXPLICIT, spec=M.GridSpec.from_c(spec), n_time=N_TIME)
         for typ in (0, 1):
             sel = np.where(sub.option_type == typ)[0]
-            _, _, _, s = O.price_batch(sub.take(sel), True, cfg, n_threads=threads)
+            gr, _, _, s = O.price_batch(sub.take(sel), True, cfg, n_threads=threads)
             solves += s
+            if c < n_chains:
+                worst = max(worst, float(np.max(np.abs(gr[:, 0] - greeks[idx[sel], 0]))))
         done += len(idx)
         c += 1
-        if time.perf_counter() - t0 > seconds:
-            break
     el = time.perf_counter() - t0
-    return {"value": done / el, "unit": "options/s", "cores": threads, "kind": "port",
-            "sample": f"{c} of {len(seg) - 1} chains ({done} options, {solves} PDE solves by the reference's "
-                      f"routing) in {el:.1f} s, oracle OpenMP batch, {threads} threads"}
+    out = {"value": done / el, "unit": "options/s", "cores": threads, "kind": "port",
+           "sample": f"{c} chain solves ({c / n_chains:.1f} passes over the {n_chains} chains, {done} options, "
+                     f"{solves} PDE solves by the reference's routing) in {el:.1f} s, oracle OpenMP batch, "
+                     f"{threads} threads",
+           "parity_max_abs_price_err_chains": worst}
+    if batched is not None:
+        gr, _, _, _ = O.price_batch(batched["_batch"], False, batched["_cfg"], use_normalized=False,
+                                    n_threads=threads)
+        out["parity_max_abs_price_err_batched"] = float(np.max(np.abs(gr[:, 0] - batched["_greeks"][:, 0])))
+    return out
 
 
 def main():
@@ -210,29 +260,34 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-options", type=int, default=65536)
     ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--batched-options", type=int, default=65536)
+    ap.add_argument("--batched-steps", type=int, default=3)
+    ap.add_argument("--batched-check", type=int, default=256, help="batched options checked against the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variant", default="", help="force lanes,nodes_per_lane of the solve kernel")
+    ap.add_argument("--variant", default="", help="force lanes,nodes_per_lane of the solve kernel at 101 points")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(options are independent; sharding is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
-    opts, seg, chain, plain = run_gpu(args)
-    parity = parity_check(opts, seg, chain["greeks"])
+    g = Gpu(args.variant)
+    chains = run_chains(g, args)
+    plain = run_plain(g, args) if args.plain_options > 0 else None
+    batched = run_batched(g, args) if args.batched_options > 0 else None
 
-    flops_launch = chain["pde_solves"] * flops_per_option(N_SPACE, N_TIME)
-    achieved = flops_launch / (chain["kernel_ms"] * 1e-3) / 1e12
+    workload = "chains_4096_101x498"
+    flops_launch = chains["pde_solves"] * float(flops_per_option(N_SPACE, N_TIME))
+    achieved = flops_launch / (chains["kernel_ms"] * 1e-3) / 1e12
     line = {
         "metric": "American options/sec @101x498 FP64 (plain vs batched); adjoint Greeks/sec",
-        "value": chain["options_per_s"],
+        "value": chains["options_per_s"],
         "unit": "options/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": chain["ms_per_step"],
+        "ms_per_step": chains["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -240,27 +295,18 @@ def main():
         "data": "synthetic (seeded xoshiro256**), inputs resident in HBM",
         "config": {"workload": "shared-grid chains: 64 chains x 64 strikes = 4096 options, 101x498, "
                                "one grid per chain, one PDE per (chain, type)",
-                   "options": int(len(opts)), "pde_solves": chain["pde_solves"], "n_space": N_SPACE,
+                   "options": int(len(chains["opts"])), "pde_solves": chains["pde_solves"], "n_space": N_SPACE,
                    "n_time": N_TIME},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "american_solve_kernel (FP64 vector ALU; no MFMA)",
-                     "kernel_ms": chain["kernel_ms"], "flops_per_launch": flops_launch},
-        "parity_max_abs_price_err_vs_oracle": parity,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_summary(workload),
+                     "kernel": "american_solve_kernel (FP64 vector ALU; the path has no MFMA work)",
+                     "kernel_ms": chains["kernel_ms"], "flops_per_launch": flops_launch},
     }
     if plain is not None:
-        pf = plain["options"] * flops_per_option(N_SPACE, N_TIME)
-        pach = pf / (plain["kernel_ms"] * 1e-3) / 1e12
-        line["plain"] = {"workload": f"{plain['options']} puts of the plain configuration distribution, "
-                                     f"per-option 101-point grids x 498 steps, one PDE per option",
-                         "options_per_s": plain["options_per_s"], "ms_per_step": plain["ms_per_step"],
-                         "kernel_ms": plain["kernel_ms"],
-                         "roofline": {"achieved": pach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                                      "frac": pach / PEAK_FP64_TFLOPS}}
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(opts, seg, args.cpu_seconds)
-    else:
-        line["cpu_baseline"] = None
+        line["plain"] = {k: v for k, v in plain.items() if not k.startswith("_")}
+    if batched is not None:
+        line["batched"] = {k: v for k, v in batched.items() if not k.startswith("_")}
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(chains, batched, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

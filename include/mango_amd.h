@@ -241,8 +241,8 @@ MangoStatus mango_price_american_chains_device(const MangoDeviceOption* d_opts, 
  * by grid size on the device (one solve-kernel variant per bin), ordered by
  * estimated cost (longest first) and served by persistent work-queue kernels,
  * one per bin, running side by side. The whole sequence is captured into a
- * hipGraph on the first call for a given (buffers, n, grid, stream) and
- * replayed afterwards. Results equal mango_price_american_batch_device's.
+ * hipGraph on the first call for a given (buffers, n, grid) and replayed
+ * on `stream` afterwards. Results equal mango_price_american_batch_device's.
  * queue_cfg may be NULL (persistent work queues, graph replay). Calls share
  * the library's device scratch: do not overlap calls. */
 typedef struct { int32_t persistent; int32_t use_graph; } MangoQueueConfig;
@@ -250,6 +250,14 @@ MangoStatus mango_price_american_queue_device(const MangoDeviceOption* d_opts, u
                                               const MangoGridConfig* grid, MangoAmericanGreeks* d_out,
                                               int32_t* d_status, void* stream, const MangoQueueConfig* queue_cfg,
                                               MangoError* err);
+
+/* Per-option grid size the device entry points use for `grid` (host
+ * memory in and out): n_space[i] and n_time[i] as estimate_pde_grid
+ * (src/option/grid_spec_types.cpp:37-118) gives them under the config's
+ * mode, with n_time raised to min_time_steps in ACCURACY mode. Invalid
+ * options get 0, 0. For work accounting and load planning. */
+MangoStatus mango_plan_grid_sizes(const MangoDeviceOption* opts, uint64_t n, const MangoGridConfig* grid,
+                                  uint64_t* n_space, uint64_t* n_time, MangoError* err);
 
 /* Kernel-variant selection (the autotuner's knob). A variant is (lanes per
  * option, grid nodes per lane) and covers n_space <= lanes * nodes_per_lane.

@@ -1228,6 +1228,34 @@ MangoStatus mango_price_american_queue_device(const MangoDeviceOption* d_opts, u
     return device_queue(d_opts, n, grid, d_out, d_status, static_cast<hipStream_t>(stream), queue_cfg, err);
 }
 
+MangoStatus mango_plan_grid_sizes(const MangoDeviceOption* opts, uint64_t n, const MangoGridConfig* grid,
+                                  uint64_t* n_space, uint64_t* n_time, MangoError* err) {
+    if (n == 0) return MANGO_OK;
+    if (!opts || !grid || !n_space || !n_time) { set_err(err, MANGO_ERR_VALIDATION, "null argument"); return MANGO_ERR_VALIDATION; }
+    const Accuracy acc = to_accuracy(&grid->accuracy);
+    for (uint64_t i = 0; i < n; ++i) {
+        const MangoDeviceOption& o = opts[i];
+        MangoPricingParams p{};
+        p.spot = o.spot; p.strike = o.strike; p.maturity = o.maturity; p.volatility = o.volatility;
+        p.rate_const = o.rate; p.dividend_yield = o.dividend_yield; p.option_type = o.option_type;
+        const char* why = nullptr;
+        n_space[i] = n_time[i] = 0;
+        if (validate_params(p, &why) != MANGO_OK) continue;
+        if (grid->mode == MANGO_GRID_MODE_EXPLICIT) {
+            n_space[i] = grid->spec.n_points;
+            n_time[i] = grid->n_time;
+            continue;
+        }
+        auto [g, nt] = estimate_grid_host(p, acc);
+        n_space[i] = static_cast<uint64_t>(g.n_points);
+        if (grid->mode == MANGO_GRID_MODE_ACCURACY_FIXED_TIME)
+            n_time[i] = grid->n_time;
+        else
+            n_time[i] = static_cast<uint64_t>(std::max<int64_t>(nt, static_cast<int64_t>(grid->min_time_steps)));
+    }
+    return MANGO_OK;
+}
+
 void mango_profile_enable(int32_t enable) {
     for (auto& [a, b] : g_events) { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
     g_events.clear();

@@ -29,6 +29,7 @@ __all__ = [
     "OptionType", "GridAccuracyProfile", "GridAccuracyParams", "make_grid_accuracy", "PricingParams",
     "Dividend", "AmericanOptionResult", "american_option_price", "BatchAmericanOptionSolver", "SolverError",
     "SolverErrorCode", "GridSpec", "grid_config", "price_batch", "estimate_pde_grid", "estimate_batch_pde_grid",
+    "plan_grid_sizes",
 ]
 
 
@@ -259,6 +260,23 @@ def estimate_pde_grid(params: PricingParams, accuracy: GridAccuracyParams | None
     if st != MANGO_OK:
         raise ValueError(err.message.decode())
     return GridSpec.from_c(spec), int(nt.value)
+
+
+def plan_grid_sizes(batch: OptionBatch, grid: MangoGridConfig):
+    """mango_plan_grid_sizes: per-option (n_space, n_time) the device entry
+    points use for `grid` (0, 0 for invalid options)."""
+    lib = _abi.load_library()
+    rec = np.ascontiguousarray(batch.records())
+    n = len(rec)
+    ns = np.zeros(n, dtype=np.uint64)
+    nt = np.zeros(n, dtype=np.uint64)
+    err = MangoError()
+    u64 = C.POINTER(C.c_uint64)
+    st = lib.mango_plan_grid_sizes(rec.ctypes.data, n, C.byref(grid), ns.ctypes.data_as(u64),
+                                   nt.ctypes.data_as(u64), C.byref(err))
+    if st != MANGO_OK:
+        raise ValueError(err.message.decode())
+    return ns.astype(np.int64), nt.astype(np.int64)
 
 
 def estimate_batch_pde_grid(params: Sequence[PricingParams], accuracy: GridAccuracyParams | None = None):

@@ -173,6 +173,8 @@ SIGNATURES = {
     "mango_price_american_queue_device": (C.c_int32, [C.c_void_p, C.c_uint64, _P(MangoGridConfig), C.c_void_p,
                                                       C.c_void_p, C.c_void_p, _P(MangoQueueConfig),
                                                       _P(MangoError)]),
+    "mango_plan_grid_sizes": (C.c_int32, [C.c_void_p, C.c_uint64, _P(MangoGridConfig), _P(C.c_uint64),
+                                          _P(C.c_uint64), _P(MangoError)]),
     "mango_tuning_set": (C.c_int32, [_P(MangoVariantRule), C.c_uint64, _P(MangoError)]),
     "mango_tuning_variants": (C.c_uint64, [_P(MangoVariantRule), C.c_uint64]),
     "mango_profile_enable": (None, [C.c_int32]),

@@ -87,6 +87,33 @@ def test_host_estimate_matches_oracle():
         np.testing.assert_array_equal(M.GridSpec.from_c(spec).generate(), O.generate(spec_o))
 
 
+@pytest.mark.parametrize("mode", ["accuracy", "fixed_time", "explicit"])
+def test_plan_grid_sizes_matches_oracle(mode):
+    """Per-option (n_space, n_time) of the device planner against the
+    oracle's estimate_pde_grid under each grid mode; invalid options -> 0, 0."""
+    opts = W.c3_uneven(300, seed=11)
+    opts.volatility[7] = -0.1
+    acc = M.GridAccuracyParams(tol=1e-3, min_spatial_points=51, max_spatial_points=401, max_time_steps=2000)
+    if mode == "accuracy":
+        cfg = M.grid_config(_abi.MANGO_GRID_MODE_ACCURACY, acc, min_time_steps=100)
+    elif mode == "fixed_time":
+        cfg = M.grid_config(_abi.MANGO_GRID_MODE_ACCURACY_FIXED_TIME, acc, n_time=321)
+    else:
+        cfg = M.grid_config(_abi.MANGO_GRID_MODE_EXPLICIT, spec=M.GridSpec.sinh_spaced(-3, 3, 151, 2.0), n_time=77)
+    ns, nt = M.plan_grid_sizes(opts, cfg)
+    assert ns[7] == 0 and nt[7] == 0
+    arr, n = O.c_params(opts)
+    for i in range(n):
+        if i == 7:
+            continue
+        if mode == "explicit":
+            assert (ns[i], nt[i]) == (151, 77)
+            continue
+        spec_o, nt_o = O.estimate_pde_grid(arr[i], acc.to_c())
+        assert ns[i] == spec_o.n_points
+        assert nt[i] == (max(nt_o, 100) if mode == "accuracy" else 321)
+
+
 def test_workload_generator_is_deterministic():
     g = W.Xoshiro256ss(0)
     first = [g.next() for _ in range(3)]
