@@ -548,10 +548,52 @@ struct Solution {
     std::vector<double> snapshots;  // [n_snap][n]
 };
 
+struct StageTape;
 struct SolveOptions {
     TRBDF2 trbdf2;
     std::vector<double> snapshot_times;  // time-to-expiry values to record
+    // Replay: take each stage's locked and clamped rows from this tape (a
+    // forward run of the same grid) instead of deciding them; the test
+    // harness uses it for finite differences of the frozen-set map.
+    const StageTape* replay = nullptr;
 };
+
+// Record of one implicit stage of the forward march, kept for a reverse
+// (adjoint) sweep: the stage matrix after Dirichlet rows and deep-ITM locks,
+// the right-hand side actually solved, the Thomas internals (c', d') and the
+// projected solution before the boundary values are re-applied.
+struct StageRecord {
+    int kind = 0;  // 0: implicit Euler half step, 1: TR-BDF2 stage 1, 2: stage 2
+    double t = 0.0, w = 0.0;
+    std::vector<double> lower, diag, upper, rhs, cp, dp, sol;
+    std::vector<char> free_row;  // interior row that is not locked
+    std::vector<char> clamped;   // back substitution took the obstacle
+    std::vector<double> u_n;     // stage 1: u^n, the input of rhs = u^n + w1 L(u^n)
+};
+struct StageTape {
+    std::vector<StageRecord> stages;
+};
+
+// Projected Thomas with the clamped rows given instead of decided: the map
+// whose derivative the adjoint computes (frozen exercise set).
+inline bool solve_thomas_forced(const double* lower, const double* diag, const double* upper, const double* rhs,
+                                const double* psi, const char* clamped, double* sol, size_t n, double* ws) {
+    double* cp = ws;
+    double* dp = ws + n;
+    if (n < 2 || std::abs(diag[0]) < kSingularTol) return false;
+    cp[0] = upper[0] / diag[0];
+    dp[0] = rhs[0] / diag[0];
+    for (size_t i = 1; i < n; ++i) {
+        const double denom = std::fma(-lower[i - 1], cp[i - 1], diag[i]);
+        if (std::abs(denom) < kSingularTol) return false;
+        if (i + 1 < n) cp[i] = upper[i] / denom;
+        dp[i] = std::fma(-lower[i - 1], dp[i - 1], rhs[i]) / denom;
+    }
+    sol[n - 1] = clamped[n - 1] ? psi[n - 1] : dp[n - 1];
+    for (size_t i = n - 1; i > 0; --i)
+        sol[i - 1] = clamped[i - 1] ? psi[i - 1] : std::fma(-cp[i - 1], sol[i], dp[i - 1]);
+    return true;
+}
 
 // convert_times_to_indices, uniform branch (src/pde/core/grid.hpp:754-836)
 inline bool convert_times_to_indices(const std::vector<double>& times, const TimeDomain& td,
@@ -576,7 +618,7 @@ inline bool convert_times_to_indices(const std::vector<double>& times, const Tim
 // AmericanOptionSolver::solve (src/option/american_option.cpp:354-438) driving
 // PDESolver::initialize/solve (src/pde/internal/pde_solver.hpp:96-178).
 inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, const TimeDomain& td,
-                              Solution& out, const SolveOptions& opts = {}) {
+                              Solution& out, const SolveOptions& opts = {}, StageTape* tape = nullptr) {
     if (!validate_pricing_params(p)) return kValidation;
     const std::vector<double> x = spec.generate();
     const size_t n = x.size();
@@ -655,7 +697,18 @@ inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, cons
     double psi_max = *std::max_element(psi.begin(), psi.end());
 
     // solve_implicit_stage_projected (pde_solver.hpp:525-729)
+    int tape_kind = 0;
+    const std::vector<double>* tape_un = nullptr;
+    std::vector<char> locked(n, 0);
+    size_t stage_index = 0;
     auto stage = [&](double t, double w, std::vector<double>& u, const std::vector<double>& rhs) -> bool {
+        std::fill(locked.begin(), locked.end(), 0);
+        const StageRecord* rp = nullptr;
+        if (opts.replay) {
+            if (stage_index >= opts.replay->stages.size()) return false;
+            rp = &opts.replay->stages[stage_index];
+        }
+        ++stage_index;
         apply_bc(u, t);
         for (size_t i = 1; i < n - 1; ++i) {
             lower[i - 1] = -w * jl[i];
@@ -667,7 +720,14 @@ inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, cons
         rhs_bc = rhs;
         rhs_bc[0] = g_left(t);
         rhs_bc[n - 1] = g_right(t);
-        if (psi_max > 0.0) {
+        if (rp) {
+            for (size_t i = 1; i < n - 1; ++i) {
+                if (!rp->free_row[i]) {
+                    lower[i - 1] = 0.0; diag[i] = 1.0; upper[i] = 0.0; rhs_bc[i] = psi[i];
+                    locked[i] = 1;
+                }
+            }
+        } else if (psi_max > 0.0) {
             const double thr = 0.95 * psi_max;
             size_t first = n - 1;
             for (size_t i = 1; i < n - 1; ++i) {
@@ -678,13 +738,38 @@ inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, cons
                 for (size_t i = first; i < n - 1; ++i) {
                     if (psi[i] > thr && u[i] - psi[i] < 1e-8 && lpsi[i] < 0.0) {
                         lower[i - 1] = 0.0; diag[i] = 1.0; upper[i] = 0.0; rhs_bc[i] = psi[i];
+                        locked[i] = 1;
                     }
                 }
             }
         }
-        if (!solve_thomas_projected(lower.data(), diag.data(), upper.data(), rhs_bc.data(),
-                                    psi.data(), u.data(), n, ws.data()))
+        if (rp) {
+            if (!solve_thomas_forced(lower.data(), diag.data(), upper.data(), rhs_bc.data(), psi.data(),
+                                     rp->clamped.data(), u.data(), n, ws.data()))
+                return false;
+        } else if (!solve_thomas_projected(lower.data(), diag.data(), upper.data(), rhs_bc.data(),
+                                           psi.data(), u.data(), n, ws.data())) {
             return false;
+        }
+        if (tape) {
+            StageRecord rec;
+            rec.kind = tape_kind;
+            rec.t = t;
+            rec.w = w;
+            rec.lower = lower; rec.diag = diag; rec.upper = upper; rec.rhs = rhs_bc;
+            rec.cp.assign(ws.begin(), ws.begin() + static_cast<std::ptrdiff_t>(n));
+            rec.dp.assign(ws.begin() + static_cast<std::ptrdiff_t>(n), ws.end());
+            rec.sol = u;
+            rec.free_row.assign(n, 0);
+            for (size_t i = 1; i + 1 < n; ++i) rec.free_row[i] = locked[i] ? 0 : 1;
+            rec.clamped.assign(n, 0);
+            for (size_t i = 0; i < n; ++i) {
+                const double unc = i + 1 < n ? std::fma(-rec.cp[i], u[i + 1], rec.dp[i]) : rec.dp[i];
+                rec.clamped[i] = unc < psi[i] ? 1 : 0;
+            }
+            if (tape_un) rec.u_n = *tape_un;
+            tape->stages.push_back(std::move(rec));
+        }
         apply_bc(u, t);
         return true;
     };
@@ -708,6 +793,8 @@ inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, cons
         const double t_mid = t + half_dt;
         u_prev = u;
         rhs = u_prev; u = u_prev;
+        tape_kind = 0;
+        tape_un = nullptr;
         if (!stage(t_mid, half_dt, u, rhs)) return kLinearSolveFailure;
         u_prev = u;
         rhs = u_prev; u = u_prev;
@@ -726,6 +813,8 @@ inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, cons
         apply_L(u_prev, lu);
         for (size_t i = 0; i < n; ++i) rhs[i] = std::fma(w1, lu[i], u_prev[i]);
         u = u_prev;
+        tape_kind = 1;
+        tape_un = &u_prev;
         if (!stage(t_stage1, w1, u, rhs)) return kLinearSolveFailure;
         // stage 2 (pde_solver.hpp:370-395)
         const double omg = 1.0 - gamma;
@@ -735,6 +824,8 @@ inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, cons
         const double beta = -(omg * omg) / den;
         const double w2 = (1.0 - gamma) * dt / (2.0 - gamma);
         for (size_t i = 0; i < n; ++i) rhs[i] = std::fma(alpha, u[i], beta * u_prev[i]);
+        tape_kind = 2;
+        tape_un = nullptr;
         if (!stage(t_next, w2, u, rhs)) return kLinearSolveFailure;
         t = t_next;
         record(step + 1, u);

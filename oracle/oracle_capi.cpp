@@ -4,6 +4,7 @@
 // restatement in mango_oracle.hpp so that pytest (ctypes) and bench.py's CPU
 // baseline can call it. Uses the struct layouts of include/mango_amd.h.
 #include "mango_oracle.hpp"
+#include "mango_adjoint.hpp"
 
 #include "../include/mango_amd.h"
 
@@ -207,6 +208,66 @@ uint64_t oracle_price_batch(const MangoPricingParams* p, uint64_t n, int32_t use
         }
     }
     if (pde_solves) *pde_solves = bs.pde_solves;
+    return failed;
+}
+
+// Price of `bumped` on an explicit grid with the exercise set frozen at the
+// one a forward run of `base` on the same grid takes (locked and clamped rows
+// replayed stage by stage): the map the adjoint differentiates.
+int32_t oracle_solve_frozen(const MangoPricingParams* base, const MangoPricingParams* bumped,
+                            const MangoGridSpec* spec, uint64_t n_time, MangoAmericanGreeks* greeks) {
+    const auto pb = to_params(*base);
+    const auto pq = to_params(*bumped);
+    const auto g = to_spec(*spec);
+    Solution s;
+    StageTape tape;
+    int32_t st = solve_american(pb, g, TimeDomain::from_n_steps(0.0, pb.maturity, n_time), s, {}, &tape);
+    if (st != kOk) return map_status(st);
+    SolveOptions o;
+    o.replay = &tape;
+    st = solve_american(pq, g, TimeDomain::from_n_steps(0.0, pq.maturity, n_time), s, o);
+    if (st != kOk) return map_status(st);
+    const auto gk = compute_greeks(s, pq.spot, pq.strike);
+    *greeks = {gk.value, gk.delta, gk.gamma, gk.theta};
+    return MANGO_OK;
+}
+
+// Reverse-mode sensitivities (mango_adjoint.hpp) per option on the grid the
+// device entry points use for `grid`: out[i*6 .. i*6+5] = value, delta,
+// dV/dsigma, dV/dr, dV/dq, dV/dK. Returns the failed count.
+uint64_t oracle_adjoint_batch(const MangoPricingParams* p, uint64_t n, const MangoGridConfig* grid, double* out,
+                              int32_t* status, int32_t n_threads) {
+    uint64_t failed = 0;
+    #pragma omp parallel for schedule(dynamic, 4) num_threads(n_threads > 0 ? n_threads : 1) reduction(+ : failed)
+    for (uint64_t i = 0; i < n; ++i) {
+        const auto pp = to_params(p[i]);
+        int32_t st = kValidation;
+        AdjointResult a;
+        if (validate_pricing_params(pp)) {
+            GridSpec spec;
+            size_t nt;
+            if (grid->mode == MANGO_GRID_MODE_EXPLICIT) {
+                spec = to_spec(grid->spec);
+                nt = grid->n_time;
+            } else {
+                auto g = estimate_pde_grid(pp, to_acc(&grid->accuracy));
+                spec = g.first;
+                nt = grid->mode == MANGO_GRID_MODE_ACCURACY_FIXED_TIME
+                         ? grid->n_time
+                         : std::max<size_t>(g.second.n_steps, grid->min_time_steps);
+            }
+            st = adjoint_american(pp, spec, TimeDomain::from_n_steps(0.0, pp.maturity, nt), a);
+        }
+        status[i] = map_status(st);
+        double* o = out + 6 * i;
+        if (st == kOk) {
+            o[0] = a.value; o[1] = a.delta; o[2] = a.dv_dsigma; o[3] = a.dv_drate; o[4] = a.dv_ddividend;
+            o[5] = a.dv_dstrike;
+        } else {
+            for (int k = 0; k < 6; ++k) o[k] = kNaN;
+            ++failed;
+        }
+    }
     return failed;
 }
 

@@ -50,6 +50,10 @@ def load() -> C.CDLL:
         "oracle_price_batch": (C.c_uint64, [P(MangoPricingParams), C.c_uint64, C.c_int32, C.c_int32,
                                             P(MangoGridConfig), P(MangoAmericanGreeks), P(C.c_int32), C.c_int32,
                                             P(C.c_uint64)]),
+        "oracle_solve_frozen": (C.c_int32, [P(MangoPricingParams), P(MangoPricingParams), P(MangoGridSpec),
+                                            C.c_uint64, P(MangoAmericanGreeks)]),
+        "oracle_adjoint_batch": (C.c_uint64, [P(MangoPricingParams), C.c_uint64, P(MangoGridConfig), _dp,
+                                              P(C.c_int32), C.c_int32]),
         "oracle_thomas": (C.c_int32, [_dp, _dp, _dp, _dp, _dp, C.c_uint64]),
         "oracle_thomas_projected": (C.c_int32, [_dp, _dp, _dp, _dp, _dp, _dp, C.c_uint64]),
         "oracle_spline_eval": (C.c_int32, [_dp, _dp, C.c_uint64, _dp, C.c_uint64, _dp, _dp]),
@@ -176,6 +180,16 @@ def price_batch(options, use_shared_grid=False, grid: MangoGridConfig | None = N
     g = np.frombuffer(out, dtype=np.float64, count=4 * n).reshape(n, 4).copy()
     s = np.frombuffer(status, dtype=np.int32, count=n).copy()
     return g, s, int(failed), int(solves.value)
+
+
+def adjoint_batch(options, grid: MangoGridConfig, n_threads=1):
+    """Reverse-mode sensitivities on each option's frozen grid. Returns
+    (out[n,6] = value, delta, dV/dsigma, dV/dr, dV/dq, dV/dK; status[n])."""
+    arr, n = c_params(options)
+    out = np.zeros((max(1, n), 6))
+    status = (C.c_int32 * max(1, n))()
+    load().oracle_adjoint_batch(arr, n, C.byref(grid), _p(out), status, n_threads)
+    return out[:n].copy(), np.frombuffer(status, dtype=np.int32, count=n).copy()
 
 
 def operator(p: MangoPricingParams, x, u):
