@@ -251,6 +251,31 @@ MangoStatus mango_price_american_queue_device(const MangoDeviceOption* d_opts, u
                                               int32_t* d_status, void* stream, const MangoQueueConfig* queue_cfg,
                                               MangoError* err);
 
+/* Reverse-mode sensitivities of the American price on each option's frozen
+ * grid (the grid the solve uses for `grid`), with the forward's exercise
+ * sets frozen: dV/dsigma, dV/dr, dV/dq by one reverse sweep through the
+ * TR-BDF2 stages (states checkpointed every ceil(sqrt(N)) steps and
+ * recomputed per segment), dV/dK = (V - S delta)/K (strike-free PDE), plus
+ * the forward's value, delta, gamma, theta. The reference has no adjoint; the
+ * survey's batch adjoint export ("mango_price_american_batch_adjoint") is this
+ * one, and its forward is solve_american_option on that grid
+ * (src/option/american_option.cpp:325-451). Checked against central finite
+ * differences of the forward with the exercise set held fixed. */
+typedef struct {
+    double value, delta, gamma, theta;
+    double dv_dsigma, dv_drate, dv_ddividend, dv_dstrike;
+} MangoAdjointGreeks;
+/* Host memory in and out; per-slot status and a failed count like
+ * mango_price_american_batch. Synchronous. */
+MangoStatus mango_price_american_batch_adjoint(const MangoPricingParams* params, uint64_t n,
+                                               const MangoGridConfig* grid, MangoAdjointGreeks* out,
+                                               int32_t* out_status, uint64_t* out_failed, MangoError* err);
+/* Inputs and outputs in HBM (torch interop); asynchronous on `stream`. Shares
+ * the library's device scratch with the batched form: do not overlap calls. */
+MangoStatus mango_price_american_adjoint_device(const MangoDeviceOption* d_opts, uint64_t n,
+                                                const MangoGridConfig* grid, MangoAdjointGreeks* d_out,
+                                                int32_t* d_status, void* stream, MangoError* err);
+
 /* Per-option grid size the device entry points use for `grid` (host
  * memory in and out): n_space[i] and n_time[i] as estimate_pde_grid
  * (src/option/grid_spec_types.cpp:37-118) gives them under the config's

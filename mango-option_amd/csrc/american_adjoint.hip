@@ -1,0 +1,328 @@
+// SPDX-License-Identifier: MIT
+//
+// gfx950 reverse-mode (adjoint) kernel: dV/dsigma, dV/dr, dV/dq and dV/dK of
+// the American price on each option's frozen grid, with the forward's
+// exercise sets frozen (deep-ITM locked rows and rows the projected back
+// substitution clamped to the obstacle). No counterpart in the reference; the
+// forward it differentiates is the TR-BDF2 march of src/pde/internal/
+// pde_solver.hpp:96-729 on the grid of src/option/grid_spec_types.cpp:37-118.
+//
+// One option per group of K lanes, in the solve kernel's register layout.
+//   1. forward march, storing the state every S = ceil(sqrt(N)) steps
+//      (checkpoints, group scratch in HBM); Greeks at spot as in the solve;
+//   2. seed: dV/du^N = K ds(ln(S/K))/du of the natural spline (lane 0, LDS);
+//   3. segments from last to first: recompute the segment from its
+//      checkpoint storing u^n and the stage-1 state per step (group scratch),
+//      then reverse its steps: stage 2, then stage 1 (Rannacher's two Euler
+//      half steps for step 0). Each stage reverse is three cross-lane affine
+//      scans (back-substitution carry, rhs adjoint, denominator adjoint);
+//   4. the per-node adjoints of the Jacobian bands are contracted with the
+//      stencil weights (L = a d2 + b d1 + c I, a = sigma^2/2,
+//      b = r - q - sigma^2/2, c = -r) and summed over the group:
+//      dV/dsigma = sigma (dV/da - dV/db), dV/dr = dV/db - dV/dc + boundary,
+//      dV/dq = -dV/db; dV/dK = (V - S Delta)/K (strike-free PDE).
+// Cost: one forward, one recompute and one reverse march (about three stage
+// solves' worth of scans per step).
+#include "american_device.hpp"
+
+namespace mango_amd {
+namespace {
+
+// dV/dy of V = K max(0, s(ln(S/K))) for the natural spline s through (x, y)
+// (cubic_spline_solver.hpp:74-370; value clamped to the boundary values
+// outside the grid, american_option_result.cpp:29-49). Serial, one lane;
+// cps and cs are scratch rows.
+__device__ void value_seed(const double* x, const double* y, int n, double spot, double strike, double* cps,
+                           double* cs, double* ybar) {
+    for (int i = 0; i < n; ++i) ybar[i] = 0.0;
+    const double xe = log(spot / strike);
+    if (xe <= x[0]) { ybar[0] = strike; return; }
+    if (xe >= x[n - 1]) { ybar[n - 1] = strike; return; }
+    if (!spline_second(x, y, n, cps, cs)) return;
+    if (spline_eval(x, y, cs, n, xe) <= 0.0) return;
+    const int i = find_interval(x, n, xe);
+    const double h = x[i + 1] - x[i];
+    const double dx = xe - x[i];
+    ybar[i] += strike * (1.0 - dx / h);
+    ybar[i + 1] += strike * (dx / h);
+    if (n <= 2) return;
+    // ds/dc of knots i and i+1; the interior knots solve M c = R y with M
+    // symmetric tridiagonal (diag 2(h_{k-1} + h_k), off-diagonal h_k)
+    for (int k = 0; k < n; ++k) cs[k] = 0.0;
+    if (i >= 1) cs[i] = strike * (-2.0 * h * dx / 3.0 + dx * dx - dx * dx * dx / (3.0 * h));
+    if (i + 1 <= n - 2) cs[i + 1] = strike * (-h * dx / 3.0 + dx * dx * dx / (3.0 * h));
+    // z = M^-1 cbar in place in cs[1..n-2] (cps holds c')
+    {
+        const double h0 = x[1] - x[0], h1 = x[2] - x[1];
+        const double d0 = 2.0 * (h0 + h1);
+        cps[1] = h1 / d0;
+        cs[1] = cs[1] / d0;
+        for (int k = 2; k <= n - 2; ++k) {
+            const double hl = x[k] - x[k - 1], hr = x[k + 1] - x[k];
+            const double den = 2.0 * (hl + hr) - hl * cps[k - 1];
+            cps[k] = hr / den;
+            cs[k] = (cs[k] - hl * cs[k - 1]) / den;
+        }
+        for (int k = n - 3; k >= 1; --k) cs[k] -= cps[k] * cs[k + 1];
+    }
+    // rhs_k = 3((y_{k+1} - y_k)/h_k - (y_k - y_{k-1})/h_{k-1}): ybar += R^T z
+    for (int k = 1; k <= n - 2; ++k) {
+        const double hl = x[k] - x[k - 1], hr = x[k + 1] - x[k];
+        const double z = cs[k];
+        ybar[k + 1] += 3.0 * z / hr;
+        ybar[k] -= 3.0 * z * (1.0 / hr + 1.0 / hl);
+        ybar[k - 1] += 3.0 * z / hl;
+    }
+}
+
+template <int K, int M>
+__device__ __forceinline__ void row_store(double* __restrict__ row, int lane, const double (&v)[M]) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) row[k * K + lane] = v[k];
+}
+template <int K, int M>
+__device__ __forceinline__ void row_load(const double* __restrict__ row, int lane, double (&v)[M]) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) v[k] = row[k * K + lane];
+}
+
+template <int K, int M>
+__device__ __forceinline__ void adjoint_one(const PdeJob* __restrict__ jobs, int job_index, int lane,
+                                            double* __restrict__ xs, double* __restrict__ ys, double* __restrict__ cps,
+                                            double* __restrict__ cs, double* __restrict__ yb, double* __restrict__ scr,
+                                            int seg_cap, int ck_cap, AdjointGreeks* __restrict__ out,
+                                            int32_t* __restrict__ status_out) {
+    constexpr int NMAX = K * M;
+    const PdeJob job = jobs[job_index];
+    OptionPde<K, M> P;
+    P.init(job, lane, xs);
+    const int n = P.n;
+    const int base = P.base;
+    const int64_t N = P.steps(job);
+    const double dt = job.maturity / static_cast<double>(N);
+    int64_t S = static_cast<int64_t>(ceil(sqrt(static_cast<double>(N))));
+    if (S < 1) S = 1;
+    const int64_t n_ck = (N + S - 1) / S;
+    bool ok = P.valid && N > 0 && S <= seg_cap && n_ck <= ck_cap;
+    double* ck = scr;
+    double* tr = scr + static_cast<size_t>(ck_cap) * NMAX;
+
+    // ---- 1. forward with checkpoints ----
+    StageCache<M> c0, c1;
+    c0.invalidate();
+    c1.invalidate();
+    double u[M], mid[M], up[M];
+    P.initial(u);
+#pragma unroll
+    for (int k = 0; k < M; ++k) up[k] = mid[k] = u[k];
+    for (int64_t s = 0; ok && s < N; ++s) {
+        if (s % S == 0) row_store<K, M>(ck + (s / S) * NMAX, lane, u);
+#pragma unroll
+        for (int k = 0; k < M; ++k) up[k] = u[k];
+        ok = P.advance(s, dt, u, mid, c0, c1);
+    }
+    if (N == 1) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) up[k] = mid[k];  // the Rannacher start leaves its half step as u_prev
+    }
+
+    // ---- Greeks at spot (as the solve kernel) and the seed dV/du^N ----
+#pragma unroll
+    for (int k = 0; k < M; ++k) { const int i = base + k; if (i < n && P.valid) ys[i] = u[k]; }
+    group_sync();
+    AdjointGreeks res{NAN, NAN, NAN, NAN, NAN, NAN, NAN, NAN};
+    double v_cur = 0.0;
+    if (lane == 0 && ok) {
+        spline_second(xs, ys, n, cps, cs);
+        const double xsp = log(job.spot / job.strike);
+        v_cur = spline_eval(xs, ys, cs, n, xsp);
+        const Greeks t = greeks_at(xs, ys, cs, ys, cs, n, P.uniform, job.spot, job.strike, dt);
+        res.value = t.value; res.delta = t.delta; res.gamma = t.gamma;
+        value_seed(xs, ys, n, job.spot, job.strike, cps, cs, yb);
+    }
+    group_sync();
+    double ub[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) { const int i = base + k; ub[k] = (ok && i < n) ? yb[i] : 0.0; }
+#pragma unroll
+    for (int k = 0; k < M; ++k) { const int i = base + k; if (i < n && P.valid) ys[i] = up[k]; }
+    group_sync();
+    if (lane == 0 && ok) {
+        spline_second(xs, ys, n, cps, cs);
+        const double v_prev = spline_eval(xs, ys, cs, n, log(job.spot / job.strike));
+        res.theta = (v_prev - v_cur) / dt * job.strike;
+    }
+
+    // ---- 3. segments in reverse: recompute, then reverse the steps ----
+    const TrConstants trc;
+    double gl[M], gd[M], gu[M], unext[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) { gl[k] = gd[k] = gu[k] = 0.0; unext[k] = u[k]; }
+    double rbc = 0.0;
+    for (int64_t seg = n_ck - 1; ok && seg >= 0; --seg) {
+        const int64_t s0 = seg * S;
+        const int64_t s1 = (s0 + S < N) ? s0 + S : N;
+        double w[M];
+        row_load<K, M>(ck + seg * NMAX, lane, w);
+        for (int64_t s = s0; ok && s < s1; ++s) {
+            row_store<K, M>(tr + (2 * (s - s0)) * NMAX, lane, w);
+            ok = P.advance(s, dt, w, mid, c0, c1);
+            row_store<K, M>(tr + (2 * (s - s0) + 1) * NMAX, lane, mid);
+        }
+        for (int64_t s = s1 - 1; ok && s >= s0; --s) {
+            double un[M], um[M], rhs[M], rbf[M], unb[M];
+            row_load<K, M>(tr + (2 * (s - s0)) * NMAX, lane, un);
+            row_load<K, M>(tr + (2 * (s - s0) + 1) * NMAX, lane, um);
+            if (s == 0) {
+                const double half_dt = 0.5 * dt;
+                c0.invalidate();
+                ok = P.stage_reverse(0.0 + dt, half_dt, um, um, c0, unext, ub, rbf, gl, gd, gu, rbc);
+                ok = ok && P.stage_reverse(0.0 + half_dt, half_dt, un, un, c0, um, rbf, unb, gl, gd, gu, rbc);
+                c0.invalidate();
+            } else {
+                const double t = static_cast<double>(s) * dt;
+                const double t_next = static_cast<double>(s + 1) * dt;
+                const double w1 = trc.w1(dt);
+#pragma unroll
+                for (int k = 0; k < M; ++k) rhs[k] = fma(trc.alpha, um[k], trc.beta * un[k]);
+                ok = P.stage_reverse(t_next, trc.w2(dt), rhs, um, c1, unext, ub, rbf, gl, gd, gu, rbc);
+                double umb[M];
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    umb[k] = trc.alpha * rbf[k];
+                    unb[k] = trc.beta * rbf[k];
+                }
+                P.stage1_rhs(w1, un, rhs);
+                ok = ok && P.stage_reverse(t + trc.gamma * dt, w1, rhs, un, c0, um, umb, rbf, gl, gd, gu, rbc);
+                P.stage1_rhs_reverse(w1, un, rbf, unb, gl, gd, gu);
+            }
+#pragma unroll
+            for (int k = 0; k < M; ++k) { ub[k] = unb[k]; unext[k] = un[k]; }
+        }
+    }
+
+    // ---- 4. contract with the stencil weights ----
+    double sa = 0.0, sb = 0.0, sc = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const int i = base + k;
+        if (ok && i >= 1 && i <= n - 2) {
+            const double dl = xs[i] - xs[i - 1], dr = xs[i + 1] - xs[i], dxa = (dl + dr) / 2.0;
+            const double d2l = 1.0 / (dl * dxa), d2d = -(1.0 / dl + 1.0 / dr) / dxa, d2u = 1.0 / (dr * dxa);
+            const double d1l = -dr / (dl * (dl + dr)), d1d = (dr - dl) / (dl * dr), d1u = dl / (dr * (dl + dr));
+            sa += gl[k] * d2l + gd[k] * d2d + gu[k] * d2u;
+            sb += gl[k] * d1l + gd[k] * d1d + gu[k] * d1u;
+            sc += gd[k];
+        }
+    }
+    sa = group_sum<K>(sa);
+    sb = group_sum<K>(sb);
+    sc = group_sum<K>(sc);
+    rbc = group_sum<K>(rbc);
+    if (lane == 0) {
+        int32_t st = kStatusOk;
+        if (!P.valid) st = params_valid(job) ? kStatusSolver : kStatusValidation;
+        else if (!ok) st = kStatusSolver;
+        if (st == kStatusOk) {
+            res.dv_dsigma = job.volatility * (sa - sb);
+            res.dv_drate = sb - sc + rbc;
+            res.dv_ddividend = -sb;
+            res.dv_dstrike = (res.value - job.spot * res.delta) / job.strike;
+        } else {
+            res = AdjointGreeks{NAN, NAN, NAN, NAN, NAN, NAN, NAN, NAN};
+        }
+        out[job.out_index] = res;
+        status_out[job.out_index] = st;
+    }
+    group_sync();
+}
+
+// Persistent groups pull job slots from *queue (one counter per launch);
+// group g of block b owns scratch slot b * GPB + g.
+template <int K, int M>
+__global__ __launch_bounds__(kSolveBlock) void american_adjoint_kernel(
+    const PdeJob* __restrict__ jobs, int n_jobs, const int32_t* __restrict__ d_n_jobs,
+    const int32_t* __restrict__ job_list, const int32_t* __restrict__ d_list_offset, int32_t* __restrict__ queue,
+    double* __restrict__ scratch, int seg_cap, int ck_cap, AdjointGreeks* __restrict__ out,
+    int32_t* __restrict__ status_out) {
+    constexpr int NMAX = K * M;
+    constexpr int GPB = kSolveBlock / K;
+    if (job_list && d_list_offset) job_list += *d_list_offset;
+    __shared__ double s_x[GPB][NMAX];
+    __shared__ double s_y[GPB][NMAX];
+    __shared__ double s_cp[GPB][NMAX];
+    __shared__ double s_c[GPB][NMAX];
+    __shared__ double s_yb[GPB][NMAX];
+    if (d_n_jobs) n_jobs = min(n_jobs, *d_n_jobs);
+    const int slot = threadIdx.x / K;
+    const int lane = threadIdx.x % K;
+    double* scr = scratch + (static_cast<size_t>(blockIdx.x) * GPB + slot) *
+                                static_cast<size_t>(ck_cap + 2 * seg_cap) * NMAX;
+    for (;;) {
+        // branch-free fetch: every lane takes part, the group's lane 0 adds 1
+        const int t = atomicAdd(queue, lane == 0 ? 1 : 0);
+        const int j = __shfl(t, 0, K);
+        if (j >= n_jobs) break;
+        adjoint_one<K, M>(jobs, job_list ? job_list[j] : j, lane, s_x[slot], s_y[slot], s_cp[slot], s_c[slot],
+                          s_yb[slot], scr, seg_cap, ck_cap, out, status_out);
+    }
+}
+
+template <int K, int M>
+int resident_blocks() {
+    int per_cu = 0, dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, american_adjoint_kernel<K, M>, kSolveBlock, 0);
+    return std::max(1, per_cu) * std::max(1, cus);
+}
+
+}  // namespace
+
+#define MANGO_ADJOINT_VARIANTS(X) X(32, 4) X(64, 4) X(64, 7)
+
+SolveVariant pick_adjoint_variant(int64_t n_space) {
+    if (n_space <= 128) return {32, 4};
+    if (n_space <= 256) return {64, 4};
+    return {64, 7};
+}
+
+int max_adjoint_nodes() { return 64 * 7; }
+
+void adjoint_rows(int64_t max_steps, int* seg_cap, int* ck_cap) {
+    const int64_t s = static_cast<int64_t>(std::ceil(std::sqrt(static_cast<double>(std::max<int64_t>(max_steps, 1)))));
+    *seg_cap = static_cast<int>(s);
+    *ck_cap = static_cast<int>(s + 1);  // ceil(N / ceil(sqrt(N))) <= ceil(sqrt(N)) + 1
+}
+
+int adjoint_groups(const SolveVariant& v, int n_jobs) {
+#define MANGO_GROUPS(K, M)                                                    \
+    if (v.lanes == K && v.nodes == M) {                                       \
+        constexpr int GPB = kSolveBlock / K;                                  \
+        const int blocks = std::min((n_jobs + GPB - 1) / GPB, resident_blocks<K, M>()); \
+        return std::max(blocks, 1) * GPB;                                     \
+    }
+    MANGO_ADJOINT_VARIANTS(MANGO_GROUPS)
+#undef MANGO_GROUPS
+    return 0;
+}
+
+int launch_american_adjoint(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, const int32_t* d_n_jobs,
+                            const int32_t* job_list, const int32_t* d_list_offset, int32_t* queue, double* d_scratch,
+                            int seg_cap, int ck_cap, AdjointGreeks* d_out, int32_t* d_status, hipStream_t stream) {
+    if (n_jobs <= 0) return 0;
+#define MANGO_LAUNCH(K, M)                                                                                     \
+    if (v.lanes == K && v.nodes == M) {                                                                        \
+        constexpr int GPB = kSolveBlock / K;                                                                   \
+        const int blocks = adjoint_groups(v, n_jobs) / GPB;                                                    \
+        hipLaunchKernelGGL((american_adjoint_kernel<K, M>), dim3(blocks), dim3(kSolveBlock), 0, stream, d_jobs, \
+                           n_jobs, d_n_jobs, job_list, d_list_offset, queue, d_scratch, seg_cap, ck_cap, d_out,   \
+                           d_status);                                                                          \
+        return hipGetLastError() == hipSuccess ? 0 : 1;                                                        \
+    }
+    MANGO_ADJOINT_VARIANTS(MANGO_LAUNCH)
+#undef MANGO_LAUNCH
+    return 2;
+}
+
+}  // namespace mango_amd

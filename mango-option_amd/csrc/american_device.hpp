@@ -1,0 +1,922 @@
+// SPDX-License-Identifier: MIT
+//
+// Device building blocks shared by the gfx950 solve and adjoint kernels:
+// serial spline and Greeks code run by one lane on LDS rows, the DPP
+// cross-lane scans of affine / clamped-affine maps, and group primitives.
+// Included by each kernel translation unit (internal linkage).
+#pragma once
+
+#include "american_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+
+namespace mango_amd {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSolveBlock = 64;  // one wavefront per block
+constexpr double kBig = 1e300;
+constexpr double kSingularTol = 1e-15;   // ThomasConfig::singularity_tol
+constexpr double kDeepItmFraction = 0.95; // pde_solver.hpp:635
+constexpr double kExerciseTol = 1e-8;     // pde_solver.hpp:636
+
+// std::max / std::min semantics (first argument returned on ties and NaN).
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double clampd(double y, double lo, double hi) { return smin(smax(y, lo), hi); }
+
+template <int K>
+__device__ __forceinline__ double shup(double v, int d) { return __shfl_up(v, static_cast<unsigned>(d), K); }
+template <int K>
+__device__ __forceinline__ double shdn(double v, int d) { return __shfl_down(v, static_cast<unsigned>(d), K); }
+template <int K>
+__device__ __forceinline__ double group_max(double v) {
+#pragma unroll
+    for (int o = K / 2; o > 0; o >>= 1) v = smax(v, __shfl_xor(v, o, K));
+    return v;
+}
+template <int K>
+__device__ __forceinline__ double group_min(double v) {
+#pragma unroll
+    for (int o = K / 2; o > 0; o >>= 1) v = smin(v, __shfl_xor(v, o, K));
+    return v;
+}
+template <int K>
+__device__ __forceinline__ bool group_any(bool p) {
+    int v = p ? 1 : 0;
+#pragma unroll
+    for (int o = K / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, K);
+    return v != 0;
+}
+
+// Exact power-of-two rescale of a 2x2 matrix (ratios are what matter).
+__device__ __forceinline__ void renorm(double& a, double& b, double& c, double& d) {
+    const double m = fmax(fmax(fabs(a), fabs(b)), fmax(fabs(c), fabs(d)));
+    if (m > 0.0 && m < kBig) {
+        int e;
+        (void)frexp(m, &e);
+        a = ldexp(a, -e); b = ldexp(b, -e); c = ldexp(c, -e); d = ldexp(d, -e);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Serial pieces run by one lane on LDS rows (epilogue): natural cubic spline
+// (src/math/cubic_spline_solver.hpp:74-370) and the at-spot Greeks of
+// AmericanOptionResult (src/option/american_option_result.cpp:25-165).
+// ---------------------------------------------------------------------------
+// c[0..n-1] of the natural spline through (x, y); cp is scratch of n.
+__device__ bool spline_second(const double* x, const double* y, int n, double* cp, double* c) {
+    c[0] = 0.0;
+    c[n - 1] = 0.0;
+    if (n <= 2) return true;
+    const int m = n - 2;
+    // row i <-> knot k = i + 1; c[k] holds d'_i then the solution
+    auto h = [&](int k) { return x[k + 1] - x[k]; };
+    auto diag = [&](int i) { return 2.0 * (h(i) + h(i + 1)); };
+    auto rhs = [&](int i) {
+        const int k = i + 1;
+        const double slope_right = (y[k + 1] - y[k]) / h(k);
+        const double slope_left = (y[k] - y[k - 1]) / h(k - 1);
+        return 3.0 * (slope_right - slope_left);
+    };
+    if (m == 1) {
+        const double d0 = diag(0);
+        if (fabs(d0) < kSingularTol) return false;
+        c[1] = rhs(0) / d0;
+        return true;
+    }
+    {
+        const double d0 = diag(0);
+        if (fabs(d0) < kSingularTol) return false;
+        cp[1] = h(1) / d0;  // upper[0] = h[1]
+        c[1] = rhs(0) / d0;
+    }
+    for (int i = 1; i < m - 1; ++i) {
+        const double lo = h(i);
+        const double denom = fma(-lo, cp[i], diag(i));
+        if (fabs(denom) < kSingularTol) return false;
+        const double inv = 1.0 / denom;
+        cp[i + 1] = h(i + 1) * inv;
+        c[i + 1] = fma(-lo, c[i], rhs(i)) * inv;
+    }
+    {
+        const int i = m - 1;
+        const double lo = h(i);
+        const double denom = fma(-lo, cp[i], diag(i));
+        if (fabs(denom) < kSingularTol) return false;
+        c[i + 1] = fma(-lo, c[i], rhs(i)) / denom;
+    }
+    for (int i = m - 1; i > 0; --i) c[i] = fma(-cp[i], c[i + 1], c[i]);
+    return true;
+}
+
+__device__ int find_interval(const double* x, int n, double xe) {
+    if (xe <= x[0]) return 0;
+    if (xe >= x[n - 1]) return n - 2;
+    int lo = 0, hi = n;  // lower_bound: first idx with x[idx] >= xe
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (x[mid] < xe) lo = mid + 1; else hi = mid;
+    }
+    int idx = lo;
+    if (idx > 0 && (idx == n || x[idx] > xe)) --idx;
+    return idx < n - 2 ? idx : n - 2;
+}
+
+__device__ void spline_coeffs(const double* x, const double* y, const double* c, int i,
+                              double& a, double& b, double& ci, double& d) {
+    const double h = x[i + 1] - x[i];
+    a = y[i];
+    b = (y[i + 1] - y[i]) / h - h * (2.0 * c[i] + c[i + 1]) / 3.0;
+    ci = c[i];
+    d = (c[i + 1] - c[i]) / (3.0 * h);
+}
+
+__device__ double spline_eval(const double* x, const double* y, const double* c, int n, double xe) {
+    const int i = find_interval(x, n, xe);
+    double a, b, ci, d;
+    spline_coeffs(x, y, c, i, a, b, ci, d);
+    const double dx = xe - x[i];
+    return a + b * dx + ci * (dx * dx) + d * (dx * dx * dx);
+}
+
+__device__ double spline_deriv(const double* x, const double* y, const double* c, int n, double xe) {
+    const int i = find_interval(x, n, xe);
+    double a, b, ci, d;
+    spline_coeffs(x, y, c, i, a, b, ci, d);
+    const double dx = xe - x[i];
+    return b + 2.0 * ci * dx + 3.0 * d * (dx * dx);
+}
+
+__device__ bool grid_uniform(const double* x, int n) {
+    const double expected = (x[n - 1] - x[0]) / static_cast<double>(n - 1);
+    const double tol = 100.0 * 2.220446049250313e-16;
+    for (int i = 1; i < n; ++i)
+        if (fabs((x[i] - x[i - 1]) - expected) > tol) return false;
+    return true;
+}
+
+// CenteredDifference at node i (1 <= i <= n-2) (centered_difference.hpp:26-99).
+__device__ void stencil_at(const double* x, const double* u, int n, bool uniform, int i,
+                           double& d1, double& d2) {
+    if (uniform) {
+        const double dx = (x[n - 1] - x[0]) / static_cast<double>(n - 1);
+        const double dx_inv = 1.0 / dx;
+        const double dx2_inv = dx_inv * dx_inv;
+        const double half_dx_inv = dx_inv * 0.5;
+        d2 = fma(u[i + 1] + u[i - 1], dx2_inv, -2.0 * u[i] * dx2_inv);
+        d1 = (u[i + 1] - u[i - 1]) * half_dx_inv;
+    } else {
+        const double dl = x[i] - x[i - 1];
+        const double dr = x[i + 1] - x[i];
+        const double dc = 0.5 * (dl + dr);
+        const double dxl_inv = 1.0 / dl, dxr_inv = 1.0 / dr, dxc_inv = 1.0 / dc;
+        const double wl = dr / (dl + dr), wr = dl / (dl + dr);
+        const double fwd = (u[i + 1] - u[i]) * dxr_inv;
+        const double bwd = (u[i] - u[i - 1]) * dxl_inv;
+        d2 = (fwd - bwd) * dxc_inv;
+        const double t1 = wl * (u[i] - u[i - 1]) * dxl_inv;
+        const double t2 = wr * (u[i + 1] - u[i]) * dxr_inv;
+        d1 = t1 + t2;
+    }
+}
+
+// value/delta/gamma at spot from (x, u, c_u); theta also needs (u_prev, c_up).
+__device__ Greeks greeks_at(const double* x, const double* u, const double* cu, const double* up,
+                            const double* cup, int n, bool uniform, double S, double K, double dt) {
+    Greeks g;
+    const double xs = log(S / K);
+    const double v_spline = spline_eval(x, u, cu, n, xs);
+    if (xs <= x[0]) g.value = u[0] * K;
+    else if (xs >= x[n - 1]) g.value = u[n - 1] * K;
+    else g.value = smax(0.0, v_spline) * K;
+    g.delta = spline_deriv(x, u, cu, n, xs) * (K / S);
+    // gamma: stencil values exist on [1, n-2) only; the rest read as zero
+    int il, ir;
+    if (xs <= x[0]) { il = ir = 0; }
+    else if (xs >= x[n - 1]) { il = ir = n - 1; }
+    else {
+        int lo = 0, hi = n;
+        while (lo < hi) { const int mid = (lo + hi) >> 1; if (x[mid] < xs) lo = mid + 1; else hi = mid; }
+        ir = lo;
+        il = ir > 0 ? ir - 1 : 0;
+        if (fabs(x[ir] - xs) < 1e-14) il = ir;
+    }
+    double d1l = 0.0, d2l = 0.0, d1r = 0.0, d2r = 0.0;
+    if (il >= 1 && il < n - 2) stencil_at(x, u, n, uniform, il, d1l, d2l);
+    if (ir >= 1 && ir < n - 2) stencil_at(x, u, n, uniform, ir, d1r, d2r);
+    double dvs, d2vs;
+    if (il == ir) { dvs = d1l; d2vs = d2l; }
+    else {
+        const double al = (xs - x[il]) / (x[ir] - x[il]);
+        dvs = d1l + al * (d1r - d1l);
+        d2vs = d2l + al * (d2r - d2l);
+    }
+    const double k_over_s2 = K / (S * S);
+    g.gamma = fma(k_over_s2, d2vs, -k_over_s2 * dvs);
+    const double v_prev = spline_eval(x, up, cup, n, xs);
+    g.theta = (v_prev - v_spline) / dt * K;
+    return g;
+}
+
+__device__ bool params_valid(const PdeJob& j) {
+    auto fin = [](double v) { return isfinite(v); };
+    if (!(j.spot > 0.0) || !fin(j.spot)) return false;
+    if (!(j.strike > 0.0) || !fin(j.strike)) return false;
+    if (!(j.maturity > 0.0) || !fin(j.maturity)) return false;
+    if (!fin(j.rate)) return false;
+    if (j.dividend_yield < 0.0 || !fin(j.dividend_yield)) return false;
+    if (!(j.volatility > 0.0) || !fin(j.volatility)) return false;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Cross-lane building blocks. A group of K lanes spans K/16 DPP rows. Inside
+// a row DPP row_shr/row_shl move values at VALU latency; across rows the
+// prefix direction uses row_bcast and the suffix direction an indexed
+// shuffle of the next row's first lane.
+// ---------------------------------------------------------------------------
+constexpr int kRowShl = 0x100;
+constexpr int kRowShr = 0x110;
+constexpr int kWaveShl1 = 0x130;
+constexpr int kWaveShr1 = 0x138;
+constexpr int kRowBcast15 = 0x142;
+constexpr int kRowBcast31 = 0x143;
+
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp(double old, double v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xf, false);
+}
+// Lanes whose source is outside the row read 0 (no identity fill needed).
+template <int CTRL>
+__device__ __forceinline__ double dpp0(double v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
+}
+
+// Affine map x -> A + B x (forward-elimination recurrence of d').
+struct Aff {
+    double A, B;
+};
+__device__ __forceinline__ Aff compose(const Aff& f, const Aff& q) { return {fma(f.B, q.A, f.A), f.B * q.B}; }
+
+// Clamped-affine map v -> min(max(a v + b, lo), hi) (projected back substitution).
+struct Clamp {
+    double a, b, lo, hi;
+};
+__device__ __forceinline__ Clamp clamp_identity() { return {1.0, 0.0, -kBig, kBig}; }
+// f o q. For either sign of f.a the image of [q.lo, q.hi] is [min(t1,t2), max(t1,t2)].
+__device__ __forceinline__ Clamp compose(const Clamp& f, const Clamp& q) {
+    const double t1 = fma(f.a, q.lo, f.b);
+    const double t2 = fma(f.a, q.hi, f.b);
+    return {f.a * q.a, fma(f.a, q.b, f.b), fmin(fmax(fmin(t1, t2), f.lo), f.hi),
+            fmin(fmax(fmax(t1, t2), f.lo), f.hi)};
+}
+// f_k(v) = max(m v + d, L) composed on the left of q.
+__device__ __forceinline__ Clamp compose_row(double m, double d, double L, const Clamp& q) {
+    const double t1 = fma(m, q.lo, d);
+    const double t2 = fma(m, q.hi, d);
+    return {m * q.a, fma(m, q.b, d), fmax(fmin(t1, t2), L), fmax(fmax(t1, t2), L)};
+}
+
+// Non-decreasing clamp v -> max(a v + b, lo), a >= 0: the form every row map
+// takes when all c' <= 0 (the usual case: then -c' >= 0).
+struct Mono {
+    double a, b, lo;
+};
+__device__ __forceinline__ Mono compose(const Mono& f, const Mono& q) {
+    return {f.a * q.a, fma(f.a, q.b, f.b), fmax(fma(f.a, q.lo, f.b), f.lo)};
+}
+__device__ __forceinline__ Mono compose_row_mono(double m, double d, double L, const Mono& q) {
+    return {m * q.a, fma(m, q.b, d), fmax(fma(m, q.lo, d), L)};
+}
+
+// Inclusive prefix over the lanes of a group, lower lanes applied first:
+// lane j ends with c_j o c_{j-1} o ... o c_0.
+template <int K>
+__device__ __forceinline__ Clamp scan_prefix(Clamp c) {
+#define MANGO_PREFIX_STEP(CTRL, ROWS)                                                             \
+    {                                                                                             \
+        const Clamp p{dpp<CTRL, ROWS>(1.0, c.a), dpp<CTRL, ROWS>(0.0, c.b), dpp<CTRL, ROWS>(-kBig, c.lo), \
+                      dpp<CTRL, ROWS>(kBig, c.hi)};                                               \
+        c = compose(c, p);                                                                        \
+    }
+    if constexpr (K > 1) MANGO_PREFIX_STEP(kRowShr + 1, 0xf)
+    if constexpr (K > 2) MANGO_PREFIX_STEP(kRowShr + 2, 0xf)
+    if constexpr (K > 4) MANGO_PREFIX_STEP(kRowShr + 4, 0xf)
+    if constexpr (K > 8) MANGO_PREFIX_STEP(kRowShr + 8, 0xf)
+    if constexpr (K > 16) MANGO_PREFIX_STEP(kRowBcast15, 0xa)
+    if constexpr (K > 32) MANGO_PREFIX_STEP(kRowBcast31, 0xc)
+#undef MANGO_PREFIX_STEP
+    return c;
+}
+
+// Same for monotone maps. With K = 16 the whole group is one DPP row, and a
+// lane whose in-row source does not exist already holds its complete prefix,
+// which is a constant map (it contains the top node, where c' = 0): composing
+// it with the zero map read by bound_ctrl keeps its value at every v.
+template <int K>
+__device__ __forceinline__ Mono scan_prefix(Mono c) {
+    if constexpr (K == 16) {
+#define MANGO_STEP0(D) { const Mono p{dpp0<kRowShr + D>(c.a), dpp0<kRowShr + D>(c.b), dpp0<kRowShr + D>(c.lo)}; c = compose(c, p); }
+        MANGO_STEP0(1) MANGO_STEP0(2) MANGO_STEP0(4) MANGO_STEP0(8)
+#undef MANGO_STEP0
+    } else {
+#define MANGO_STEP(CTRL, ROWS) { const Mono p{dpp<CTRL, ROWS>(1.0, c.a), dpp<CTRL, ROWS>(0.0, c.b), dpp<CTRL, ROWS>(-kBig, c.lo)}; c = compose(c, p); }
+        if constexpr (K > 1) MANGO_STEP(kRowShr + 1, 0xf)
+        if constexpr (K > 2) MANGO_STEP(kRowShr + 2, 0xf)
+        if constexpr (K > 4) MANGO_STEP(kRowShr + 4, 0xf)
+        if constexpr (K > 8) MANGO_STEP(kRowShr + 8, 0xf)
+        if constexpr (K > 16) MANGO_STEP(kRowBcast15, 0xa)
+        if constexpr (K > 32) MANGO_STEP(kRowBcast31, 0xc)
+#undef MANGO_STEP
+    }
+    return c;
+}
+
+// Inclusive suffix over the lanes of a group, higher lanes applied first:
+// lane j ends with c_j o c_{j+1} o ... o c_{K-1}.
+// Value of a double in wavefront lane L (uniform result).
+__device__ __forceinline__ double readlane_d(double v, int L) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffll), L);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), L);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+template <int K>
+__device__ __forceinline__ Aff scan_suffix(Aff c, int j) {
+    if constexpr (K == 16) {
+        // out-of-row lanes read (0, 0); such a lane already holds its complete
+        // suffix, which contains node 0 (beta = 0), so its B is 0 and A is kept.
+#define MANGO_STEP0(D) { const Aff p{dpp0<kRowShl + D>(c.A), dpp0<kRowShl + D>(c.B)}; c = compose(c, p); }
+        MANGO_STEP0(1) MANGO_STEP0(2) MANGO_STEP0(4) MANGO_STEP0(8)
+#undef MANGO_STEP0
+        return c;
+    } else {
+#define MANGO_STEP(D) { const Aff p{dpp<kRowShl + D>(0.0, c.A), dpp<kRowShl + D>(1.0, c.B)}; c = compose(c, p); }
+        MANGO_STEP(1) MANGO_STEP(2) MANGO_STEP(4) MANGO_STEP(8)
+#undef MANGO_STEP
+        // Row crossing: the first lane of each row now holds its row's
+        // aggregate; read them as scalars and give each row the composition
+        // of all rows after it within the group.
+        const int wl = __lane_id();
+        if constexpr (K == 32) {
+            const Aff a1{readlane_d(c.A, 16), readlane_d(c.B, 16)};
+            const Aff a3{readlane_d(c.A, 48), readlane_d(c.B, 48)};
+            const Aff nxt = (wl < 32) ? a1 : a3;
+            if ((j & 16) == 0) c = compose(c, nxt);
+        } else {
+            const Aff a1{readlane_d(c.A, 16), readlane_d(c.B, 16)};
+            const Aff a2{readlane_d(c.A, 32), readlane_d(c.B, 32)};
+            const Aff a3{readlane_d(c.A, 48), readlane_d(c.B, 48)};
+            const Aff s2 = compose(a2, a3);
+            const Aff s1 = compose(a1, s2);
+            const int row = j >> 4;
+            if (row < 3) c = compose(c, row == 0 ? s1 : (row == 1 ? s2 : a3));
+        }
+        return c;
+    }
+}
+
+template <int K>
+__device__ __forceinline__ uint64_t group_lanes() {
+    if constexpr (K == 64) return ~0ull;
+    else return ((1ull << K) - 1ull) << ((__lane_id() / K) * K);
+}
+
+// ---------------------------------------------------------------------------
+// The solve kernel. Lane j of a group holds grid nodes
+// [(K-1-j) M, (K-1-j) M + M): lane K-1 holds node 0, lane 0 the top nodes
+// (and the padding beyond n-1). In this order the back substitution (from
+// node n-1 down to 0) is a prefix over lanes and the forward elimination a
+// suffix.
+// ---------------------------------------------------------------------------
+// Wave-scope ordering of LDS traffic: a group never spans two wavefronts.
+__device__ __forceinline__ void group_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+
+// Inclusive prefix of affine maps over the lanes of a group, lower lanes
+// applied first: lane j ends with c_j o c_{j-1} o ... o c_0. With K = 16 an
+// out-of-row source reads the zero map, which is correct when the value
+// entering the top of the group is zero (the lane's prefix is then complete).
+template <int K>
+__device__ __forceinline__ Aff scan_prefix(Aff c) {
+    if constexpr (K == 16) {
+#define MANGO_STEP0(D) { const Aff p{dpp0<kRowShr + D>(c.A), dpp0<kRowShr + D>(c.B)}; c = compose(c, p); }
+        MANGO_STEP0(1) MANGO_STEP0(2) MANGO_STEP0(4) MANGO_STEP0(8)
+#undef MANGO_STEP0
+    } else {
+#define MANGO_STEP(CTRL, ROWS) { const Aff p{dpp<CTRL, ROWS>(0.0, c.A), dpp<CTRL, ROWS>(1.0, c.B)}; c = compose(c, p); }
+        MANGO_STEP(kRowShr + 1, 0xf) MANGO_STEP(kRowShr + 2, 0xf) MANGO_STEP(kRowShr + 4, 0xf)
+        MANGO_STEP(kRowShr + 8, 0xf)
+        if constexpr (K > 16) MANGO_STEP(kRowBcast15, 0xa)
+        if constexpr (K > 32) MANGO_STEP(kRowBcast31, 0xc)
+#undef MANGO_STEP
+    }
+    return c;
+}
+
+template <int K>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int o = K / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, K);
+    return v;
+}
+
+// c' and 1/denom of one stage weight for the current lock pattern (the
+// factorization depends only on the bands, i.e. on weight and locks).
+template <int M>
+struct StageCache {
+    double cp[M], iv[M];
+    uint32_t mask;
+    bool sing, mono;
+    __device__ __forceinline__ void invalidate() {
+        mask = ~0u;
+        sing = false;
+        mono = false;
+    }
+};
+
+// TR-BDF2 constants (trbdf2_config.hpp:17-49; pde_solver.hpp:342-395).
+struct TrConstants {
+    double gamma, alpha, beta;
+    __device__ __forceinline__ TrConstants() {
+        gamma = 2.0 - sqrt(2.0);
+        const double omg = 1.0 - gamma;
+        const double tmg = 2.0 - gamma;
+        const double den = gamma * tmg;
+        alpha = 1.0 / den;
+        beta = -(omg * omg) / den;
+    }
+    __device__ __forceinline__ double w1(double dt) const { return gamma * dt / 2.0; }
+    __device__ __forceinline__ double w2(double dt) const { return (1.0 - gamma) * dt / (2.0 - gamma); }
+};
+
+// One option on the K lanes of a group (lane j holds nodes [(K-1-j) M, +M)):
+// the per-option constants of the log-space Black-Scholes operator on its
+// grid, the forward projected stage, and the stage's reverse with the
+// exercise sets frozen. The forward follows the same operation order as the
+// plain solve kernel.
+template <int K, int M>
+struct OptionPde {
+    int lane, n, base, k_right;
+    bool put, valid, uniform;
+    uint64_t gmask;
+    double r, q, hss, drift, expected_dx, psi_max, dx_min, g_left, x_last;
+    double psi[M], jl[M], jd[M], ju[M];
+    uint32_t lockable;
+
+    // Grid into xs (LDS, n doubles), obstacle, Jacobian bands of L
+    // (spatial_operator.hpp:123-164), deep-ITM lock eligibility
+    // (pde_solver.hpp:631-680) and Dirichlet values (american_option.cpp:192-202, 276-288).
+    __device__ __forceinline__ void init(const PdeJob& job, int lane_, double* __restrict__ xs) {
+        constexpr int NMAX = K * M;
+        lane = lane_;
+        n = static_cast<int>(job.grid.n_points);
+        put = job.option_type == 1;
+        valid = params_valid(job) && n >= 3 && n <= NMAX;
+        base = (K - 1 - lane) * M;
+        gmask = group_lanes<K>();
+        if (valid) {
+            const GridDesc g = job.grid;
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                const int i = base + k;
+                if (i < n) xs[i] = raw_grid_point(g, i);
+            }
+        }
+        group_sync();
+        if (valid && lane == 0 && needs_monotonicity(job.grid)) enforce_monotonicity(xs, n, job.grid.x_min, job.grid.x_max);
+        group_sync();
+        r = job.rate;
+        q = job.dividend_yield;
+        hss = 0.5 * job.volatility * job.volatility;
+        drift = r - q - hss;
+        lockable = 0;
+        bool nonuni = false;
+        double my_psi_max = -kBig, my_dx_min = kBig;
+        expected_dx = valid ? (xs[n - 1] - xs[0]) / static_cast<double>(n - 1) : 1.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            psi[k] = -kBig;
+            jl[k] = jd[k] = ju[k] = 0.0;
+            if (valid && i < n) {
+                psi[k] = payoff(xs[i]);
+                my_psi_max = smax(my_psi_max, psi[k]);
+                if (i >= 1) {
+                    const double dx = xs[i] - xs[i - 1];
+                    my_dx_min = smin(my_dx_min, dx);
+                    if (fabs(dx - expected_dx) > 100.0 * 2.220446049250313e-16) nonuni = true;
+                }
+                if (i >= 1 && i <= n - 2) {
+                    const double dl = xs[i] - xs[i - 1];
+                    const double dr = xs[i + 1] - xs[i];
+                    const double a = hss, b = r - q - hss, c = -r;
+                    const double dx_avg = (dl + dr) / 2.0;
+                    const double d2_im1 = a / (dl * dx_avg);
+                    const double d2_i = -a * (1.0 / dl + 1.0 / dr) / dx_avg;
+                    const double d2_ip1 = a / (dr * dx_avg);
+                    const double d1_denom = dl + dr;
+                    const double d1_im1 = -b * dr / (dl * d1_denom);
+                    const double d1_i = b * (dr - dl) / (dl * dr);
+                    const double d1_ip1 = b * dl / (dr * d1_denom);
+                    jl[k] = d2_im1 + d1_im1;
+                    jd[k] = d2_i + d1_i + c;
+                    ju[k] = d2_ip1 + d1_ip1;
+                }
+            }
+        }
+        uniform = (__ballot(nonuni) & gmask) == 0;
+        psi_max = group_max<K>(my_psi_max);
+        dx_min = group_min<K>(my_dx_min);
+        {
+            const double thr = kDeepItmFraction * psi_max;
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                const int i = base + k;
+                if (valid && psi_max > 0.0 && i >= 1 && i <= n - 2 && psi[k] > thr) {
+                    double pv[3] = {payoff(xs[i - 1]), psi[k], payoff(xs[i + 1])};
+                    double d1, d2;
+                    stencil_at(xs + (i - 1), pv, 3, false, 1, d1, d2);
+                    if (uniform) {
+                        const double dx = expected_dx;
+                        const double dx_inv = 1.0 / dx;
+                        const double dx2_inv = dx_inv * dx_inv;
+                        d2 = fma(pv[2] + pv[0], dx2_inv, -2.0 * pv[1] * dx2_inv);
+                        d1 = (pv[2] - pv[0]) * (dx_inv * 0.5);
+                    }
+                    if (hss * d2 + drift * d1 - r * pv[1] < 0.0) lockable |= (1u << k);
+                }
+            }
+        }
+        const double x_first = valid ? xs[0] : 0.0;
+        x_last = valid ? xs[n - 1] : 0.0;
+        g_left = put ? smax(1.0 - exp(x_first), 0.0) : 0.0;
+        k_right = (n - 1 >= base && n - 1 < base + M) ? n - 1 - base : -1;
+    }
+
+    // Value held by lane + 1 (the node block just below) / lane - 1 (just
+    // above), 0 at the group's edge. The DPP move is issued by every lane and
+    // only then selected: under a lane-dependent branch the source lane could
+    // be inactive, and a DPP read of an inactive lane returns the old value.
+    __device__ __forceinline__ double from_below(double v) const {
+        const double x = dpp<kWaveShl1>(0.0, v);
+        return lane == K - 1 ? 0.0 : x;
+    }
+    __device__ __forceinline__ double from_above(double v) const {
+        const double x = dpp<kWaveShr1>(0.0, v);
+        return lane == 0 ? 0.0 : x;
+    }
+
+    __device__ __forceinline__ double payoff(double xv) const {
+        return put ? smax(1.0 - exp(xv), 0.0) : smax(exp(xv) - 1.0, 0.0);
+    }
+    __device__ __forceinline__ double g_right(double t) const { return put ? 0.0 : exp(x_last) - exp(-r * t); }
+    // d g_right / dr (the call's right boundary)
+    __device__ __forceinline__ double g_right_dr(double t) const { return put ? 0.0 : t * exp(-r * t); }
+
+    // TimeDomain::from_n_steps; auto step count (grid_spec_types.cpp:90-99)
+    __device__ __forceinline__ int64_t steps(const PdeJob& job) const {
+        int64_t n_steps = job.n_time;
+        if (n_steps <= 0) {
+            const double dt_est = job.c_t * dx_min;
+            int64_t nt = static_cast<int64_t>(ceil(job.maturity / dt_est));
+            if (nt > job.max_time_steps) nt = job.max_time_steps;
+            if (nt < job.min_time_steps) nt = job.min_time_steps;
+            n_steps = nt;
+        }
+        return n_steps;
+    }
+
+    // initial condition (pde_solver.hpp:96-109)
+    __device__ __forceinline__ void initial(double (&u)[M]) const {
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            double v = psi[k];
+            if (i == 0) v = g_left;
+            if (i == n - 1) v = g_right(0.0);
+            u[k] = (i < n) ? smax(v, psi[k]) : 0.0;
+        }
+    }
+
+    // Stage matrix rows (Dirichlet rows, deep-ITM locks), c'/1/denom from the
+    // cache (refreshed when the lock pattern changed) and d'.
+    __device__ __forceinline__ bool eliminate(double t, double w, const double (&rhs)[M], const double (&guess)[M],
+                                              StageCache<M>& c, double (&nlo)[M], double (&dp)[M], uint32_t& mask,
+                                              double& gr) const {
+        gr = g_right(t);
+        double rb[M];
+        mask = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const bool lock = ((lockable >> k) & 1u) && (guess[k] - psi[k] < kExerciseTol);
+            if (lock) mask |= (1u << k);
+            nlo[k] = lock ? 0.0 : w * jl[k];
+            double v = lock ? psi[k] : rhs[k];
+            if (k == k_right) v = gr;
+            rb[k] = v;
+        }
+        if (lane == K - 1) rb[0] = g_left;
+        if ((__ballot(mask != c.mask) & gmask) != 0) {
+            double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
+            double dg[M], ub[M];
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                const int i = base + k;
+                const bool active = i >= 1 && i <= n - 2 && !((mask >> k) & 1u);
+                dg[k] = active ? 1.0 - w * jd[k] : 1.0;
+                ub[k] = active ? -w * ju[k] : 0.0;
+                const double n00 = ub[k] * m10;
+                const double n01 = ub[k] * m11;
+                const double n10 = fma(nlo[k], m00, dg[k] * m10);
+                const double n11 = fma(nlo[k], m01, dg[k] * m11);
+                m00 = n00; m01 = n01; m10 = n10; m11 = n11;
+                renorm(m00, m01, m10, m11);
+            }
+#pragma unroll
+            for (int d = 1; d < K; d <<= 1) {
+                const double p00 = shdn<K>(m00, d), p01 = shdn<K>(m01, d);
+                const double p10 = shdn<K>(m10, d), p11 = shdn<K>(m11, d);
+                if (lane + d < K) {
+                    const double a00 = fma(m00, p00, m01 * p10);
+                    const double a01 = fma(m00, p01, m01 * p11);
+                    const double a10 = fma(m10, p00, m11 * p10);
+                    const double a11 = fma(m10, p01, m11 * p11);
+                    m00 = a00; m01 = a01; m10 = a10; m11 = a11;
+                    renorm(m00, m01, m10, m11);
+                }
+            }
+            const double e01 = shdn<K>(m01, 1);
+            const double e11 = shdn<K>(m11, 1);
+            double cprev = (lane == K - 1) ? 0.0 : e01 / e11;
+            bool sing = false;
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                const double denom = fma(nlo[k], cprev, dg[k]);
+                if (fabs(denom) < kSingularTol) sing = true;
+                const double inv = 1.0 / denom;
+                c.cp[k] = ub[k] * inv;
+                c.iv[k] = inv;
+                cprev = c.cp[k];
+            }
+            c.sing = (__ballot(sing) & gmask) != 0;
+            bool pos = false;
+#pragma unroll
+            for (int k = 0; k < M; ++k) pos = pos || (c.cp[k] > 0.0);
+            c.mono = (__ballot(pos) & gmask) == 0;
+            c.mask = mask;
+        }
+        if (c.sing) return false;
+        Aff f{0.0, 1.0};
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const double al = rb[k] * c.iv[k];
+            const double be = nlo[k] * c.iv[k];
+            f = Aff{fma(be, f.A, al), be * f.B};
+        }
+        f = scan_suffix<K>(f, lane);
+        double dprev = dpp<kWaveShl1>(0.0, f.A);
+        if (lane == K - 1) dprev = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            dp[k] = fma(nlo[k], dprev, rb[k]) * c.iv[k];
+            dprev = dp[k];
+        }
+        return true;
+    }
+
+    // One projected implicit stage (solve_implicit_stage_projected,
+    // pde_solver.hpp:525-729): u = projected solve, then the boundary values.
+    __device__ __forceinline__ bool stage(double t, double w, const double (&rhs)[M], const double (&guess)[M],
+                                          StageCache<M>& c, double (&u)[M]) const {
+        double nlo[M], dp[M], gr;
+        uint32_t mask;
+        if (!eliminate(t, w, rhs, guess, c, nlo, dp, mask, gr)) return false;
+        double v;
+        if (c.mono) {
+            Mono F{1.0, 0.0, -kBig};
+#pragma unroll
+            for (int k = M - 1; k >= 0; --k) F = compose_row_mono(-c.cp[k], dp[k], psi[k], F);
+            F = scan_prefix<K>(F);
+            const double eb = dpp<kWaveShr1>(0.0, F.b);
+            const double elo = dpp<kWaveShr1>(-kBig, F.lo);
+            v = (lane == 0) ? 0.0 : fmax(eb, elo);
+        } else {
+            Clamp F = clamp_identity();
+#pragma unroll
+            for (int k = M - 1; k >= 0; --k) F = compose_row(-c.cp[k], dp[k], psi[k], F);
+            F = scan_prefix<K>(F);
+            const double eb = dpp<kWaveShr1>(0.0, F.b);
+            const double elo = dpp<kWaveShr1>(-kBig, F.lo);
+            const double ehi = dpp<kWaveShr1>(kBig, F.hi);
+            v = (lane == 0) ? 0.0 : fmin(fmax(eb, elo), ehi);
+        }
+#pragma unroll
+        for (int k = M - 1; k >= 0; --k) {
+            double uk = fmax(fma(-c.cp[k], v, dp[k]), psi[k]);
+            v = uk;
+            if (k == k_right) uk = gr;
+            u[k] = uk;
+        }
+        if (lane == K - 1) u[0] = g_left;
+        return true;
+    }
+
+    // Stage-1 right-hand side u + w1 L(u), L(u) = J u (pde_solver.hpp:342-362).
+    __device__ __forceinline__ void stage1_rhs(double w1, const double (&un)[M], double (&rhs)[M]) const {
+        const double below = dpp<kWaveShl1>(0.0, un[M - 1]);
+        const double above = dpp<kWaveShr1>(0.0, un[0]);
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const double ul = (k == 0) ? below : un[k > 0 ? k - 1 : 0];
+            const double ur = (k == M - 1) ? above : un[k < M - 1 ? k + 1 : 0];
+            const double lu = fma(jl[k], ul, fma(ju[k], ur, jd[k] * un[k]));
+            rhs[k] = fma(w1, lu, un[k]);
+        }
+    }
+
+    // Time step `step` of n: step 0 is the Rannacher start (two implicit-Euler
+    // half steps, pde_solver.hpp:413-447), later steps TR-BDF2. u is advanced
+    // in place; mid receives the intermediate state (half step / u*).
+    // c0 serves the Euler half steps (then is invalidated) and stage 1, c1 stage 2.
+    __device__ __forceinline__ bool advance(int64_t step, double dt, double (&u)[M], double (&mid)[M],
+                                            StageCache<M>& c0, StageCache<M>& c1) const {
+        const TrConstants tr;
+        double un[M], rhs[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) un[k] = u[k];
+        if (step == 0) {
+            const double half_dt = 0.5 * dt;
+            c0.invalidate();
+            bool ok = stage(0.0 + half_dt, half_dt, un, un, c0, u);
+#pragma unroll
+            for (int k = 0; k < M; ++k) mid[k] = u[k];
+            ok = ok && stage(0.0 + dt, half_dt, mid, mid, c0, u);
+            c0.invalidate();
+            return ok;
+        }
+        const double t = static_cast<double>(step) * dt;
+        const double t_next = static_cast<double>(step + 1) * dt;
+        const double w1 = tr.w1(dt);
+        stage1_rhs(w1, un, rhs);
+        if (!stage(t + tr.gamma * dt, w1, rhs, un, c0, u)) return false;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            mid[k] = u[k];
+            rhs[k] = fma(tr.alpha, u[k], tr.beta * un[k]);
+        }
+        return stage(t_next, tr.w2(dt), rhs, mid, c1, u);
+    }
+
+    // Reverse of one stage with the exercise sets frozen (the locked rows and
+    // the rows the back substitution clamped to the obstacle). In: the stage's
+    // rhs and guess, its output uo (after the boundary values) and the adjoint
+    // ubar of that output. Out: rbf, the adjoint of the right-hand side on free
+    // rows (zero elsewhere). Adds the stage's terms to the per-node adjoints of
+    // the Jacobian bands (gl, gd, gu) and to rbc (d/dr of the call boundary).
+    //   back substitution reversed: t_i = ubar_i + e_{i-1} t_{i-1},
+    //     e_i = clamped_i ? 0 : -c'_i, g_i = clamped_i ? 0 : t_i     (upward)
+    //   elimination reversed: delta_i = (g_i + nlo_{i+1} delta_{i+1}) / den_i,
+    //     denbar_i = -delta_i d'_i - c'_i cbar'_i / den_i,
+    //     cbar'_i = -g_i u_{i+1} + nlo_{i+1} denbar_{i+1}             (downward)
+    __device__ __forceinline__ bool stage_reverse(double t, double w, const double (&rhs)[M], const double (&guess)[M],
+                                                  StageCache<M>& c, const double (&uo)[M], const double (&ubar)[M],
+                                                  double (&rbf)[M], double (&gl)[M], double (&gd)[M], double (&gu)[M],
+                                                  double& rbc) const {
+        double nlo[M], dp[M], gr;
+        uint32_t mask;
+        if (!eliminate(t, w, rhs, guess, c, nlo, dp, mask, gr)) return false;
+        const double dgr = g_right_dr(t);
+        // the solve's own value at n-1 (the output carries the boundary value)
+        double uth[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) uth[k] = (k == k_right) ? fmax(dp[k], psi[k]) : uo[k];
+        const double uth_top = from_above(uth[0]);
+        const double nlo_top = from_above(nlo[0]);
+        double unx[M], nlx[M], e[M], ut[M];
+        uint32_t clamped = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            unx[k] = (k == M - 1) ? uth_top : uth[k < M - 1 ? k + 1 : 0];
+            nlx[k] = (k == M - 1) ? nlo_top : nlo[k < M - 1 ? k + 1 : 0];
+            const double unc = fma(-c.cp[k], unx[k], dp[k]);
+            if (unc < psi[k]) clamped |= (1u << k);
+            e[k] = ((clamped >> k) & 1u) ? 0.0 : -c.cp[k];
+            ut[k] = (i == 0 || i == n - 1) ? 0.0 : ubar[k];
+            if (k == k_right) rbc = fma(ubar[k], dgr, rbc);
+        }
+        // carry scan (node-increasing: suffix over lanes)
+        const double e_in = from_below(e[M - 1]);
+        Aff f{0.0, 1.0};
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const double be = (k == 0) ? e_in : e[k > 0 ? k - 1 : 0];
+            f = Aff{fma(be, f.A, ut[k]), be * f.B};
+        }
+        f = scan_suffix<K>(f, lane);
+        double tprev = from_below(f.A);
+        double g[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const double be = (k == 0) ? e_in : e[k > 0 ? k - 1 : 0];
+            const double tk = fma(be, tprev, ut[k]);
+            g[k] = ((clamped >> k) & 1u) ? 0.0 : tk;
+            tprev = tk;
+        }
+        // delta scan (node-decreasing: prefix over lanes)
+        Aff F{0.0, 1.0};
+#pragma unroll
+        for (int k = M - 1; k >= 0; --k) {
+            const double A = c.iv[k] * g[k];
+            const double B = c.iv[k] * nlx[k];
+            F = Aff{fma(B, F.A, A), B * F.B};
+        }
+        F = scan_prefix<K>(F);
+        double din = from_above(F.A);
+        double dl[M];
+#pragma unroll
+        for (int k = M - 1; k >= 0; --k) {
+            dl[k] = c.iv[k] * fma(nlx[k], din, g[k]);
+            din = dl[k];
+        }
+        // denominator-adjoint scan (node-decreasing)
+        double hA[M], hB[M];
+        Aff H{0.0, 1.0};
+#pragma unroll
+        for (int k = M - 1; k >= 0; --k) {
+            const double ci = c.cp[k] * c.iv[k];
+            hA[k] = fma(ci * g[k], unx[k], -dl[k] * dp[k]);
+            hB[k] = -ci * nlx[k];
+            H = Aff{fma(hB[k], H.A, hA[k]), hB[k] * H.B};
+        }
+        H = scan_prefix<K>(H);
+        double hin = from_above(H.A);
+        double db[M];
+#pragma unroll
+        for (int k = M - 1; k >= 0; --k) {
+            db[k] = fma(hB[k], hin, hA[k]);
+            hin = db[k];
+        }
+        // band adjoints of the free interior rows
+        const double cp_in = from_below(c.cp[M - 1]);
+        const double dp_in = from_below(dp[M - 1]);
+        const double db_top = from_above(db[0]);
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            const double cpv = (k == 0) ? cp_in : c.cp[k > 0 ? k - 1 : 0];
+            const double dpv = (k == 0) ? dp_in : dp[k > 0 ? k - 1 : 0];
+            const double dbn = (k == M - 1) ? db_top : db[k < M - 1 ? k + 1 : 0];
+            const double nlob = fma(db[k], cpv, dl[k] * dpv);
+            const double cpb = fma(nlx[k], dbn, -g[k] * unx[k]);
+            const double ubb = cpb * c.iv[k];
+            const bool free_row = i >= 1 && i <= n - 2 && !((mask >> k) & 1u);
+            if (free_row) {
+                gl[k] = fma(w, nlob, gl[k]);
+                gd[k] = fma(-w, db[k], gd[k]);
+                gu[k] = fma(-w, ubb, gu[k]);
+            }
+            rbf[k] = free_row ? dl[k] : 0.0;
+            if (k == k_right) rbc = fma(dl[k], dgr, rbc);
+        }
+        return true;
+    }
+
+    // Reverse of rhs = u^n + w1 J u^n on the free rows: adds to unbar and to
+    // the band adjoints.
+    __device__ __forceinline__ void stage1_rhs_reverse(double w1, const double (&un)[M], const double (&rbf)[M],
+                                                       double (&unbar)[M], double (&gl)[M], double (&gd)[M],
+                                                       double (&gu)[M]) const {
+        const double below = from_below(un[M - 1]);
+        const double above = from_above(un[0]);
+        const double lo_in = from_below(ju[M - 1] * rbf[M - 1]);
+        const double hi_in = from_above(jl[0] * rbf[0]);
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const double y = rbf[k];
+            const double from_lo = (k == 0) ? lo_in : ju[k > 0 ? k - 1 : 0] * rbf[k > 0 ? k - 1 : 0];
+            const double from_hi = (k == M - 1) ? hi_in : jl[k < M - 1 ? k + 1 : 0] * rbf[k < M - 1 ? k + 1 : 0];
+            unbar[k] += y + w1 * (fma(jd[k], y, from_lo) + from_hi);
+            const double ul = (k == 0) ? below : un[k > 0 ? k - 1 : 0];
+            const double ur = (k == M - 1) ? above : un[k < M - 1 ? k + 1 : 0];
+            const double wy = w1 * y;
+            gl[k] = fma(wy, ul, gl[k]);
+            gd[k] = fma(wy, un[k], gd[k]);
+            gu[k] = fma(wy, ur, gu[k]);
+        }
+    }
+};
+
+}  // namespace
+}  // namespace mango_amd

@@ -24,401 +24,10 @@
 // Within a lane every value is produced by the reference's own expression
 // (same fma placement); only the value entering a lane's block comes from a
 // scan, i.e. is reassociated (differences at the 1e-16 relative level).
-#include "american_kernels.hpp"
-
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstdint>
+#include "american_device.hpp"
 
 namespace mango_amd {
 namespace {
-
-constexpr int kBlock = 256;
-constexpr int kSolveBlock = 64;  // one wavefront per block
-constexpr double kBig = 1e300;
-constexpr double kSingularTol = 1e-15;   // ThomasConfig::singularity_tol
-constexpr double kDeepItmFraction = 0.95; // pde_solver.hpp:635
-constexpr double kExerciseTol = 1e-8;     // pde_solver.hpp:636
-
-// std::max / std::min semantics (first argument returned on ties and NaN).
-__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
-__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double clampd(double y, double lo, double hi) { return smin(smax(y, lo), hi); }
-
-template <int K>
-__device__ __forceinline__ double shup(double v, int d) { return __shfl_up(v, static_cast<unsigned>(d), K); }
-template <int K>
-__device__ __forceinline__ double shdn(double v, int d) { return __shfl_down(v, static_cast<unsigned>(d), K); }
-template <int K>
-__device__ __forceinline__ double group_max(double v) {
-#pragma unroll
-    for (int o = K / 2; o > 0; o >>= 1) v = smax(v, __shfl_xor(v, o, K));
-    return v;
-}
-template <int K>
-__device__ __forceinline__ double group_min(double v) {
-#pragma unroll
-    for (int o = K / 2; o > 0; o >>= 1) v = smin(v, __shfl_xor(v, o, K));
-    return v;
-}
-template <int K>
-__device__ __forceinline__ bool group_any(bool p) {
-    int v = p ? 1 : 0;
-#pragma unroll
-    for (int o = K / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, K);
-    return v != 0;
-}
-
-// Exact power-of-two rescale of a 2x2 matrix (ratios are what matter).
-__device__ __forceinline__ void renorm(double& a, double& b, double& c, double& d) {
-    const double m = fmax(fmax(fabs(a), fabs(b)), fmax(fabs(c), fabs(d)));
-    if (m > 0.0 && m < kBig) {
-        int e;
-        (void)frexp(m, &e);
-        a = ldexp(a, -e); b = ldexp(b, -e); c = ldexp(c, -e); d = ldexp(d, -e);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Serial pieces run by one lane on LDS rows (epilogue): natural cubic spline
-// (src/math/cubic_spline_solver.hpp:74-370) and the at-spot Greeks of
-// AmericanOptionResult (src/option/american_option_result.cpp:25-165).
-// ---------------------------------------------------------------------------
-// c[0..n-1] of the natural spline through (x, y); cp is scratch of n.
-__device__ bool spline_second(const double* x, const double* y, int n, double* cp, double* c) {
-    c[0] = 0.0;
-    c[n - 1] = 0.0;
-    if (n <= 2) return true;
-    const int m = n - 2;
-    // row i <-> knot k = i + 1; c[k] holds d'_i then the solution
-    auto h = [&](int k) { return x[k + 1] - x[k]; };
-    auto diag = [&](int i) { return 2.0 * (h(i) + h(i + 1)); };
-    auto rhs = [&](int i) {
-        const int k = i + 1;
-        const double slope_right = (y[k + 1] - y[k]) / h(k);
-        const double slope_left = (y[k] - y[k - 1]) / h(k - 1);
-        return 3.0 * (slope_right - slope_left);
-    };
-    if (m == 1) {
-        const double d0 = diag(0);
-        if (fabs(d0) < kSingularTol) return false;
-        c[1] = rhs(0) / d0;
-        return true;
-    }
-    {
-        const double d0 = diag(0);
-        if (fabs(d0) < kSingularTol) return false;
-        cp[1] = h(1) / d0;  // upper[0] = h[1]
-        c[1] = rhs(0) / d0;
-    }
-    for (int i = 1; i < m - 1; ++i) {
-        const double lo = h(i);
-        const double denom = fma(-lo, cp[i], diag(i));
-        if (fabs(denom) < kSingularTol) return false;
-        const double inv = 1.0 / denom;
-        cp[i + 1] = h(i + 1) * inv;
-        c[i + 1] = fma(-lo, c[i], rhs(i)) * inv;
-    }
-    {
-        const int i = m - 1;
-        const double lo = h(i);
-        const double denom = fma(-lo, cp[i], diag(i));
-        if (fabs(denom) < kSingularTol) return false;
-        c[i + 1] = fma(-lo, c[i], rhs(i)) / denom;
-    }
-    for (int i = m - 1; i > 0; --i) c[i] = fma(-cp[i], c[i + 1], c[i]);
-    return true;
-}
-
-__device__ int find_interval(const double* x, int n, double xe) {
-    if (xe <= x[0]) return 0;
-    if (xe >= x[n - 1]) return n - 2;
-    int lo = 0, hi = n;  // lower_bound: first idx with x[idx] >= xe
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (x[mid] < xe) lo = mid + 1; else hi = mid;
-    }
-    int idx = lo;
-    if (idx > 0 && (idx == n || x[idx] > xe)) --idx;
-    return idx < n - 2 ? idx : n - 2;
-}
-
-__device__ void spline_coeffs(const double* x, const double* y, const double* c, int i,
-                              double& a, double& b, double& ci, double& d) {
-    const double h = x[i + 1] - x[i];
-    a = y[i];
-    b = (y[i + 1] - y[i]) / h - h * (2.0 * c[i] + c[i + 1]) / 3.0;
-    ci = c[i];
-    d = (c[i + 1] - c[i]) / (3.0 * h);
-}
-
-__device__ double spline_eval(const double* x, const double* y, const double* c, int n, double xe) {
-    const int i = find_interval(x, n, xe);
-    double a, b, ci, d;
-    spline_coeffs(x, y, c, i, a, b, ci, d);
-    const double dx = xe - x[i];
-    return a + b * dx + ci * (dx * dx) + d * (dx * dx * dx);
-}
-
-__device__ double spline_deriv(const double* x, const double* y, const double* c, int n, double xe) {
-    const int i = find_interval(x, n, xe);
-    double a, b, ci, d;
-    spline_coeffs(x, y, c, i, a, b, ci, d);
-    const double dx = xe - x[i];
-    return b + 2.0 * ci * dx + 3.0 * d * (dx * dx);
-}
-
-__device__ bool grid_uniform(const double* x, int n) {
-    const double expected = (x[n - 1] - x[0]) / static_cast<double>(n - 1);
-    const double tol = 100.0 * 2.220446049250313e-16;
-    for (int i = 1; i < n; ++i)
-        if (fabs((x[i] - x[i - 1]) - expected) > tol) return false;
-    return true;
-}
-
-// CenteredDifference at node i (1 <= i <= n-2) (centered_difference.hpp:26-99).
-__device__ void stencil_at(const double* x, const double* u, int n, bool uniform, int i,
-                           double& d1, double& d2) {
-    if (uniform) {
-        const double dx = (x[n - 1] - x[0]) / static_cast<double>(n - 1);
-        const double dx_inv = 1.0 / dx;
-        const double dx2_inv = dx_inv * dx_inv;
-        const double half_dx_inv = dx_inv * 0.5;
-        d2 = fma(u[i + 1] + u[i - 1], dx2_inv, -2.0 * u[i] * dx2_inv);
-        d1 = (u[i + 1] - u[i - 1]) * half_dx_inv;
-    } else {
-        const double dl = x[i] - x[i - 1];
-        const double dr = x[i + 1] - x[i];
-        const double dc = 0.5 * (dl + dr);
-        const double dxl_inv = 1.0 / dl, dxr_inv = 1.0 / dr, dxc_inv = 1.0 / dc;
-        const double wl = dr / (dl + dr), wr = dl / (dl + dr);
-        const double fwd = (u[i + 1] - u[i]) * dxr_inv;
-        const double bwd = (u[i] - u[i - 1]) * dxl_inv;
-        d2 = (fwd - bwd) * dxc_inv;
-        const double t1 = wl * (u[i] - u[i - 1]) * dxl_inv;
-        const double t2 = wr * (u[i + 1] - u[i]) * dxr_inv;
-        d1 = t1 + t2;
-    }
-}
-
-// value/delta/gamma at spot from (x, u, c_u); theta also needs (u_prev, c_up).
-__device__ Greeks greeks_at(const double* x, const double* u, const double* cu, const double* up,
-                            const double* cup, int n, bool uniform, double S, double K, double dt) {
-    Greeks g;
-    const double xs = log(S / K);
-    const double v_spline = spline_eval(x, u, cu, n, xs);
-    if (xs <= x[0]) g.value = u[0] * K;
-    else if (xs >= x[n - 1]) g.value = u[n - 1] * K;
-    else g.value = smax(0.0, v_spline) * K;
-    g.delta = spline_deriv(x, u, cu, n, xs) * (K / S);
-    // gamma: stencil values exist on [1, n-2) only; the rest read as zero
-    int il, ir;
-    if (xs <= x[0]) { il = ir = 0; }
-    else if (xs >= x[n - 1]) { il = ir = n - 1; }
-    else {
-        int lo = 0, hi = n;
-        while (lo < hi) { const int mid = (lo + hi) >> 1; if (x[mid] < xs) lo = mid + 1; else hi = mid; }
-        ir = lo;
-        il = ir > 0 ? ir - 1 : 0;
-        if (fabs(x[ir] - xs) < 1e-14) il = ir;
-    }
-    double d1l = 0.0, d2l = 0.0, d1r = 0.0, d2r = 0.0;
-    if (il >= 1 && il < n - 2) stencil_at(x, u, n, uniform, il, d1l, d2l);
-    if (ir >= 1 && ir < n - 2) stencil_at(x, u, n, uniform, ir, d1r, d2r);
-    double dvs, d2vs;
-    if (il == ir) { dvs = d1l; d2vs = d2l; }
-    else {
-        const double al = (xs - x[il]) / (x[ir] - x[il]);
-        dvs = d1l + al * (d1r - d1l);
-        d2vs = d2l + al * (d2r - d2l);
-    }
-    const double k_over_s2 = K / (S * S);
-    g.gamma = fma(k_over_s2, d2vs, -k_over_s2 * dvs);
-    const double v_prev = spline_eval(x, up, cup, n, xs);
-    g.theta = (v_prev - v_spline) / dt * K;
-    return g;
-}
-
-__device__ bool params_valid(const PdeJob& j) {
-    auto fin = [](double v) { return isfinite(v); };
-    if (!(j.spot > 0.0) || !fin(j.spot)) return false;
-    if (!(j.strike > 0.0) || !fin(j.strike)) return false;
-    if (!(j.maturity > 0.0) || !fin(j.maturity)) return false;
-    if (!fin(j.rate)) return false;
-    if (j.dividend_yield < 0.0 || !fin(j.dividend_yield)) return false;
-    if (!(j.volatility > 0.0) || !fin(j.volatility)) return false;
-    return true;
-}
-
-// ---------------------------------------------------------------------------
-// Cross-lane building blocks. A group of K lanes spans K/16 DPP rows. Inside
-// a row DPP row_shr/row_shl move values at VALU latency; across rows the
-// prefix direction uses row_bcast and the suffix direction an indexed
-// shuffle of the next row's first lane.
-// ---------------------------------------------------------------------------
-constexpr int kRowShl = 0x100;
-constexpr int kRowShr = 0x110;
-constexpr int kWaveShl1 = 0x130;
-constexpr int kWaveShr1 = 0x138;
-constexpr int kRowBcast15 = 0x142;
-constexpr int kRowBcast31 = 0x143;
-
-template <int CTRL, int ROWS = 0xf>
-__device__ __forceinline__ double dpp(double old, double v) {
-    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xf, false);
-}
-// Lanes whose source is outside the row read 0 (no identity fill needed).
-template <int CTRL>
-__device__ __forceinline__ double dpp0(double v) {
-    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
-}
-
-// Affine map x -> A + B x (forward-elimination recurrence of d').
-struct Aff {
-    double A, B;
-};
-__device__ __forceinline__ Aff compose(const Aff& f, const Aff& q) { return {fma(f.B, q.A, f.A), f.B * q.B}; }
-
-// Clamped-affine map v -> min(max(a v + b, lo), hi) (projected back substitution).
-struct Clamp {
-    double a, b, lo, hi;
-};
-__device__ __forceinline__ Clamp clamp_identity() { return {1.0, 0.0, -kBig, kBig}; }
-// f o q. For either sign of f.a the image of [q.lo, q.hi] is [min(t1,t2), max(t1,t2)].
-__device__ __forceinline__ Clamp compose(const Clamp& f, const Clamp& q) {
-    const double t1 = fma(f.a, q.lo, f.b);
-    const double t2 = fma(f.a, q.hi, f.b);
-    return {f.a * q.a, fma(f.a, q.b, f.b), fmin(fmax(fmin(t1, t2), f.lo), f.hi),
-            fmin(fmax(fmax(t1, t2), f.lo), f.hi)};
-}
-// f_k(v) = max(m v + d, L) composed on the left of q.
-__device__ __forceinline__ Clamp compose_row(double m, double d, double L, const Clamp& q) {
-    const double t1 = fma(m, q.lo, d);
-    const double t2 = fma(m, q.hi, d);
-    return {m * q.a, fma(m, q.b, d), fmax(fmin(t1, t2), L), fmax(fmax(t1, t2), L)};
-}
-
-// Non-decreasing clamp v -> max(a v + b, lo), a >= 0: the form every row map
-// takes when all c' <= 0 (the usual case: then -c' >= 0).
-struct Mono {
-    double a, b, lo;
-};
-__device__ __forceinline__ Mono compose(const Mono& f, const Mono& q) {
-    return {f.a * q.a, fma(f.a, q.b, f.b), fmax(fma(f.a, q.lo, f.b), f.lo)};
-}
-__device__ __forceinline__ Mono compose_row_mono(double m, double d, double L, const Mono& q) {
-    return {m * q.a, fma(m, q.b, d), fmax(fma(m, q.lo, d), L)};
-}
-
-// Inclusive prefix over the lanes of a group, lower lanes applied first:
-// lane j ends with c_j o c_{j-1} o ... o c_0.
-template <int K>
-__device__ __forceinline__ Clamp scan_prefix(Clamp c) {
-#define MANGO_PREFIX_STEP(CTRL, ROWS)                                                             \
-    {                                                                                             \
-        const Clamp p{dpp<CTRL, ROWS>(1.0, c.a), dpp<CTRL, ROWS>(0.0, c.b), dpp<CTRL, ROWS>(-kBig, c.lo), \
-                      dpp<CTRL, ROWS>(kBig, c.hi)};                                               \
-        c = compose(c, p);                                                                        \
-    }
-    if constexpr (K > 1) MANGO_PREFIX_STEP(kRowShr + 1, 0xf)
-    if constexpr (K > 2) MANGO_PREFIX_STEP(kRowShr + 2, 0xf)
-    if constexpr (K > 4) MANGO_PREFIX_STEP(kRowShr + 4, 0xf)
-    if constexpr (K > 8) MANGO_PREFIX_STEP(kRowShr + 8, 0xf)
-    if constexpr (K > 16) MANGO_PREFIX_STEP(kRowBcast15, 0xa)
-    if constexpr (K > 32) MANGO_PREFIX_STEP(kRowBcast31, 0xc)
-#undef MANGO_PREFIX_STEP
-    return c;
-}
-
-// Same for monotone maps. With K = 16 the whole group is one DPP row, and a
-// lane whose in-row source does not exist already holds its complete prefix,
-// which is a constant map (it contains the top node, where c' = 0): composing
-// it with the zero map read by bound_ctrl keeps its value at every v.
-template <int K>
-__device__ __forceinline__ Mono scan_prefix(Mono c) {
-    if constexpr (K == 16) {
-#define MANGO_STEP0(D) { const Mono p{dpp0<kRowShr + D>(c.a), dpp0<kRowShr + D>(c.b), dpp0<kRowShr + D>(c.lo)}; c = compose(c, p); }
-        MANGO_STEP0(1) MANGO_STEP0(2) MANGO_STEP0(4) MANGO_STEP0(8)
-#undef MANGO_STEP0
-    } else {
-#define MANGO_STEP(CTRL, ROWS) { const Mono p{dpp<CTRL, ROWS>(1.0, c.a), dpp<CTRL, ROWS>(0.0, c.b), dpp<CTRL, ROWS>(-kBig, c.lo)}; c = compose(c, p); }
-        if constexpr (K > 1) MANGO_STEP(kRowShr + 1, 0xf)
-        if constexpr (K > 2) MANGO_STEP(kRowShr + 2, 0xf)
-        if constexpr (K > 4) MANGO_STEP(kRowShr + 4, 0xf)
-        if constexpr (K > 8) MANGO_STEP(kRowShr + 8, 0xf)
-        if constexpr (K > 16) MANGO_STEP(kRowBcast15, 0xa)
-        if constexpr (K > 32) MANGO_STEP(kRowBcast31, 0xc)
-#undef MANGO_STEP
-    }
-    return c;
-}
-
-// Inclusive suffix over the lanes of a group, higher lanes applied first:
-// lane j ends with c_j o c_{j+1} o ... o c_{K-1}.
-// Value of a double in wavefront lane L (uniform result).
-__device__ __forceinline__ double readlane_d(double v, int L) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffll), L);
-    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), L);
-    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
-}
-
-template <int K>
-__device__ __forceinline__ Aff scan_suffix(Aff c, int j) {
-    if constexpr (K == 16) {
-        // out-of-row lanes read (0, 0); such a lane already holds its complete
-        // suffix, which contains node 0 (beta = 0), so its B is 0 and A is kept.
-#define MANGO_STEP0(D) { const Aff p{dpp0<kRowShl + D>(c.A), dpp0<kRowShl + D>(c.B)}; c = compose(c, p); }
-        MANGO_STEP0(1) MANGO_STEP0(2) MANGO_STEP0(4) MANGO_STEP0(8)
-#undef MANGO_STEP0
-        return c;
-    } else {
-#define MANGO_STEP(D) { const Aff p{dpp<kRowShl + D>(0.0, c.A), dpp<kRowShl + D>(1.0, c.B)}; c = compose(c, p); }
-        MANGO_STEP(1) MANGO_STEP(2) MANGO_STEP(4) MANGO_STEP(8)
-#undef MANGO_STEP
-        // Row crossing: the first lane of each row now holds its row's
-        // aggregate; read them as scalars and give each row the composition
-        // of all rows after it within the group.
-        const int wl = __lane_id();
-        if constexpr (K == 32) {
-            const Aff a1{readlane_d(c.A, 16), readlane_d(c.B, 16)};
-            const Aff a3{readlane_d(c.A, 48), readlane_d(c.B, 48)};
-            const Aff nxt = (wl < 32) ? a1 : a3;
-            if ((j & 16) == 0) c = compose(c, nxt);
-        } else {
-            const Aff a1{readlane_d(c.A, 16), readlane_d(c.B, 16)};
-            const Aff a2{readlane_d(c.A, 32), readlane_d(c.B, 32)};
-            const Aff a3{readlane_d(c.A, 48), readlane_d(c.B, 48)};
-            const Aff s2 = compose(a2, a3);
-            const Aff s1 = compose(a1, s2);
-            const int row = j >> 4;
-            if (row < 3) c = compose(c, row == 0 ? s1 : (row == 1 ? s2 : a3));
-        }
-        return c;
-    }
-}
-
-template <int K>
-__device__ __forceinline__ uint64_t group_lanes() {
-    if constexpr (K == 64) return ~0ull;
-    else return ((1ull << K) - 1ull) << ((__lane_id() / K) * K);
-}
-
-// ---------------------------------------------------------------------------
-// The solve kernel. Lane j of a group holds grid nodes
-// [(K-1-j) M, (K-1-j) M + M): lane K-1 holds node 0, lane 0 the top nodes
-// (and the padding beyond n-1). In this order the back substitution (from
-// node n-1 down to 0) is a prefix over lanes and the forward elimination a
-// suffix.
-// ---------------------------------------------------------------------------
-// Wave-scope ordering of LDS traffic: a group never spans two wavefronts.
-__device__ __forceinline__ void group_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // One option, start to finish, by the K lanes of a group. xs/ys/cps/cs are
 // the group's LDS rows (NMAX doubles each).

@@ -136,4 +136,21 @@ int launch_plan_chains(const DeviceOptionRecord* d_opts, const int64_t* d_seg_of
                        const DevicePlan& plan, PdeJob* d_jobs, int32_t* d_job_count, ChainOption* d_copts,
                        int32_t* d_rep, hipStream_t stream);
 
+// ---- reverse-mode sensitivities (american_adjoint.hip) ----
+struct AdjointGreeks {  // layout of MangoAdjointGreeks
+    double value, delta, gamma, theta;
+    double dv_dsigma, dv_drate, dv_ddividend, dv_dstrike;
+};
+// Variant serving n_space in the adjoint kernel (a subset of the solve variants).
+SolveVariant pick_adjoint_variant(int64_t n_space);
+int max_adjoint_nodes();
+// Checkpoint and segment rows one group needs for at most max_steps steps.
+void adjoint_rows(int64_t max_steps, int* seg_cap, int* ck_cap);
+// Persistent groups launched for n_jobs (each owns one scratch slot of
+// (ck_cap + 2 seg_cap) * lanes * nodes doubles).
+int adjoint_groups(const SolveVariant& v, int n_jobs);
+int launch_american_adjoint(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, const int32_t* d_n_jobs,
+                            const int32_t* job_list, const int32_t* d_list_offset, int32_t* queue, double* d_scratch,
+                            int seg_cap, int ck_cap, AdjointGreeks* d_out, int32_t* d_status, hipStream_t stream);
+
 }  // namespace mango_amd

@@ -675,7 +675,8 @@ bool ensure_aux_streams() {
 }
 
 // Bins for the grid sizes a configuration can produce.
-BinTable make_bins(const MangoGridConfig& grid, std::vector<SolveVariant>& variants) {
+BinTable make_bins(const MangoGridConfig& grid, std::vector<SolveVariant>& variants,
+                   SolveVariant (*pick)(int64_t) = pick_variant, int64_t cap = max_nodes_supported()) {
     int64_t lo, hi;
     if (grid.mode == MANGO_GRID_MODE_EXPLICIT) {
         lo = hi = static_cast<int64_t>(grid.spec.n_points);
@@ -689,9 +690,8 @@ BinTable make_bins(const MangoGridConfig& grid, std::vector<SolveVariant>& varia
     hi = std::max(hi, lo);
     BinTable t{};
     variants.clear();
-    const int64_t cap = max_nodes_supported();
     for (int64_t n = lo; n <= std::min(hi, cap) && t.n_bins < kMaxBins; ++n) {
-        const SolveVariant v = pick_variant(n);
+        const SolveVariant v = pick(n);
         if (t.n_bins > 0 && variants.back().lanes == v.lanes && variants.back().nodes == v.nodes) {
             t.max_n[t.n_bins - 1] = n;
         } else {
@@ -699,7 +699,7 @@ BinTable make_bins(const MangoGridConfig& grid, std::vector<SolveVariant>& varia
             t.max_n[t.n_bins++] = n;
         }
     }
-    if (t.n_bins == 0) { variants.push_back(pick_variant(cap)); t.max_n[t.n_bins++] = cap; }
+    if (t.n_bins == 0) { variants.push_back(pick(cap)); t.max_n[t.n_bins++] = cap; }
     t.max_n[t.n_bins - 1] = std::max<int64_t>(t.max_n[t.n_bins - 1], hi);  // larger grids fail validation in-kernel
     return t;
 }
@@ -805,6 +805,93 @@ MangoStatus device_queue(const MangoDeviceOption* d_opts, uint64_t n, const Mang
     g_graphs.push_back(e);
     if (hipGraphLaunch(exec, stream) != hipSuccess) {
         set_err(err, MANGO_ERR_SOLVER, "hipGraphLaunch failed");
+        return MANGO_ERR_SOLVER;
+    }
+    return MANGO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Reverse-mode sensitivities: the batched plan (size bins, longest first),
+// one persistent adjoint kernel per bin on the caller's stream, each group
+// with its own checkpoint / segment scratch slot.
+// ---------------------------------------------------------------------------
+struct AdjointScratch {
+    double* p = nullptr;
+    size_t cap = 0;  // doubles
+    bool ensure(size_t n) {
+        if (n <= cap) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(double)) != hipSuccess) { p = nullptr; return false; }
+        cap = n;
+        return true;
+    }
+};
+AdjointScratch g_adj;
+
+MangoStatus check_device_grid(const MangoGridConfig* grid, MangoError* err) {
+    if (grid->mode == MANGO_GRID_MODE_EXPLICIT && (grid->spec.n_points < 3 || grid->n_time == 0)) {
+        set_err(err, MANGO_ERR_VALIDATION, "explicit grid needs n_points >= 3 and n_time > 0");
+        return MANGO_ERR_VALIDATION;
+    }
+    if (grid->mode == MANGO_GRID_MODE_ACCURACY_FIXED_TIME && grid->n_time == 0) {
+        set_err(err, MANGO_ERR_VALIDATION, "n_time must be > 0");
+        return MANGO_ERR_VALIDATION;
+    }
+    if (grid->mode != MANGO_GRID_MODE_EXPLICIT && grid->mode != MANGO_GRID_MODE_ACCURACY &&
+        grid->mode != MANGO_GRID_MODE_ACCURACY_FIXED_TIME) {
+        set_err(err, MANGO_ERR_VALIDATION, "unknown grid mode");
+        return MANGO_ERR_VALIDATION;
+    }
+    return MANGO_OK;
+}
+
+MangoStatus device_adjoint(const MangoDeviceOption* d_opts, uint64_t n, const MangoGridConfig* grid,
+                           MangoAdjointGreeks* d_out, int32_t* d_status, hipStream_t stream, MangoError* err) {
+    if (n == 0) return MANGO_OK;
+    if (!d_opts || !d_out || !d_status || !grid) { set_err(err, MANGO_ERR_VALIDATION, "null argument"); return MANGO_ERR_VALIDATION; }
+    if (n > static_cast<uint64_t>(std::numeric_limits<int32_t>::max() / 2)) { set_err(err, MANGO_ERR_VALIDATION, "batch too large"); return MANGO_ERR_VALIDATION; }
+    if (const MangoStatus st = check_device_grid(grid, err); st != MANGO_OK) return st;
+    if (!device_ready(err)) return MANGO_ERR_SOLVER;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n > g_queue.cap) {
+        for (auto& e : g_graphs) (void)hipGraphExecDestroy(e.exec);
+        g_graphs.clear();
+        if (!g_queue.ensure(n)) { set_err(err, MANGO_ERR_SOLVER, "hipMalloc failed"); return MANGO_ERR_SOLVER; }
+    }
+    const int ni = static_cast<int>(n);
+    DevicePlan plan{};
+    plan.mode = grid->mode;
+    plan.n_time = static_cast<int32_t>(grid->n_time);
+    plan.min_time_steps = static_cast<int64_t>(grid->min_time_steps);
+    plan.accuracy = to_accuracy(&grid->accuracy);
+    plan.spec = to_desc(grid->spec);
+    std::vector<SolveVariant> variants;
+    const BinTable bins = make_bins(*grid, variants, pick_adjoint_variant, max_adjoint_nodes());
+    const int64_t max_steps = grid->mode == MANGO_GRID_MODE_ACCURACY
+                                  ? std::max<int64_t>(static_cast<int64_t>(grid->accuracy.max_time_steps),
+                                                      static_cast<int64_t>(grid->min_time_steps))
+                                  : static_cast<int64_t>(grid->n_time);
+    int seg_cap = 0, ck_cap = 0;
+    adjoint_rows(max_steps, &seg_cap, &ck_cap);
+    size_t need = 0;
+    for (int b = 0; b < bins.n_bins; ++b) {
+        const size_t groups = static_cast<size_t>(adjoint_groups(variants[b], ni));
+        need = std::max(need, groups * static_cast<size_t>(ck_cap + 2 * seg_cap) *
+                                  static_cast<size_t>(variants[b].lanes * variants[b].nodes));
+    }
+    if (!g_adj.ensure(need)) { set_err(err, MANGO_ERR_SOLVER, "hipMalloc failed (adjoint scratch)"); return MANGO_ERR_SOLVER; }
+    bool ok = launch_fill_i32(g_queue.ints, QueueScratch::kInts, 0, stream) == 0;
+    ok = ok && launch_plan_binned(reinterpret_cast<const DeviceOptionRecord*>(d_opts), ni, plan, bins, g_queue.jobs,
+                                  g_queue.keys, g_queue.hist(), g_queue.cursor(), g_queue.bin_start(),
+                                  g_queue.bin_count(), g_queue.list, stream) == 0;
+    for (int b = 0; b < bins.n_bins && ok; ++b)
+        ok = launch_american_adjoint(variants[b], g_queue.jobs, ni, g_queue.bin_count() + b, g_queue.list,
+                                     g_queue.bin_start() + b, g_queue.queues() + b, g_adj.p, seg_cap, ck_cap,
+                                     reinterpret_cast<AdjointGreeks*>(d_out), d_status, stream) == 0;
+    if (!ok) {
+        set_err(err, MANGO_ERR_SOLVER, (std::string("HIP launch error: ") + hipGetErrorString(hipGetLastError())).c_str());
         return MANGO_ERR_SOLVER;
     }
     return MANGO_OK;
@@ -1226,6 +1313,73 @@ MangoStatus mango_price_american_queue_device(const MangoDeviceOption* d_opts, u
                                               int32_t* d_status, void* stream, const MangoQueueConfig* queue_cfg,
                                               MangoError* err) {
     return device_queue(d_opts, n, grid, d_out, d_status, static_cast<hipStream_t>(stream), queue_cfg, err);
+}
+
+MangoStatus mango_price_american_adjoint_device(const MangoDeviceOption* d_opts, uint64_t n,
+                                                const MangoGridConfig* grid, MangoAdjointGreeks* d_out,
+                                                int32_t* d_status, void* stream, MangoError* err) {
+    return device_adjoint(d_opts, n, grid, d_out, d_status, static_cast<hipStream_t>(stream), err);
+}
+
+MangoStatus mango_price_american_batch_adjoint(const MangoPricingParams* params, uint64_t n,
+                                               const MangoGridConfig* grid, MangoAdjointGreeks* out,
+                                               int32_t* out_status, uint64_t* out_failed, MangoError* err) {
+    if (n && (!params || !out || !out_status || !grid)) {
+        set_err(err, MANGO_ERR_VALIDATION, "null params, grid or output");
+        return MANGO_ERR_VALIDATION;
+    }
+    if (out_failed) *out_failed = 0;
+    if (n == 0) return MANGO_OK;
+    if (const MangoStatus st = check_device_grid(grid, err); st != MANGO_OK) return st;
+    if (!device_ready(err)) return MANGO_ERR_SOLVER;
+    // host slots the device path cannot take (yield curves, discrete
+    // dividends, invalid parameters) are answered here and left out
+    std::vector<MangoDeviceOption> recs;
+    std::vector<uint64_t> where;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    for (uint64_t i = 0; i < n; ++i) {
+        const char* why = nullptr;
+        const MangoStatus st = validate_params(params[i], &why);
+        if (st != MANGO_OK) {
+            out[i] = MangoAdjointGreeks{nan, nan, nan, nan, nan, nan, nan, nan};
+            out_status[i] = st;
+            continue;
+        }
+        MangoDeviceOption o{};
+        o.spot = params[i].spot; o.strike = params[i].strike; o.maturity = params[i].maturity;
+        o.volatility = params[i].volatility; o.rate = params[i].rate_const;
+        o.dividend_yield = params[i].dividend_yield; o.option_type = params[i].option_type;
+        recs.push_back(o);
+        where.push_back(i);
+    }
+    const size_t m = recs.size();
+    if (m) {
+        DevBuf<MangoDeviceOption> d_opts(m);
+        DevBuf<MangoAdjointGreeks> d_out(m);
+        DevBuf<int32_t> d_st(m);
+        if (!d_opts.ok() || !d_out.ok() || !d_st.ok()) { set_err(err, MANGO_ERR_SOLVER, "hipMalloc failed"); return MANGO_ERR_SOLVER; }
+        if (hipMemcpy(d_opts.p, recs.data(), m * sizeof(MangoDeviceOption), hipMemcpyHostToDevice) != hipSuccess) {
+            set_err(err, MANGO_ERR_SOLVER, "hipMemcpy failed");
+            return MANGO_ERR_SOLVER;
+        }
+        const MangoStatus st = device_adjoint(d_opts.p, m, grid, d_out.p, d_st.p, nullptr, err);
+        if (st != MANGO_OK) return st;
+        std::vector<MangoAdjointGreeks> h_out(m);
+        std::vector<int32_t> h_st(m);
+        if (hipMemcpy(h_out.data(), d_out.p, m * sizeof(MangoAdjointGreeks), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(h_st.data(), d_st.p, m * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess) {
+            set_err(err, MANGO_ERR_SOLVER, "hipMemcpy failed");
+            return MANGO_ERR_SOLVER;
+        }
+        for (size_t j = 0; j < m; ++j) {
+            out[where[j]] = h_out[j];
+            out_status[where[j]] = h_st[j];
+        }
+    }
+    uint64_t failed = 0;
+    for (uint64_t i = 0; i < n; ++i) failed += out_status[i] != MANGO_OK;
+    if (out_failed) *out_failed = failed;
+    return MANGO_OK;
 }
 
 MangoStatus mango_plan_grid_sizes(const MangoDeviceOption* opts, uint64_t n, const MangoGridConfig* grid,

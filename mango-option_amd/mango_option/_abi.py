@@ -111,6 +111,11 @@ class MangoQueueConfig(C.Structure):
     _fields_ = [("persistent", C.c_int32), ("use_graph", C.c_int32)]
 
 
+class MangoAdjointGreeks(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("value", "delta", "gamma", "theta", "dv_dsigma", "dv_drate",
+                                          "dv_ddividend", "dv_dstrike")]
+
+
 class MangoDeviceOption(C.Structure):
     _fields_ = [
         ("spot", C.c_double),
@@ -133,6 +138,7 @@ assert C.sizeof(MangoGridSpec) == 136
 assert C.sizeof(MangoGridConfig) == 216
 assert C.sizeof(MangoAmericanGreeks) == 32
 assert C.sizeof(MangoDeviceOption) == 56
+assert C.sizeof(MangoAdjointGreeks) == 64
 
 LIB_NAME = "libmango_amd.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
@@ -173,6 +179,11 @@ SIGNATURES = {
     "mango_price_american_queue_device": (C.c_int32, [C.c_void_p, C.c_uint64, _P(MangoGridConfig), C.c_void_p,
                                                       C.c_void_p, C.c_void_p, _P(MangoQueueConfig),
                                                       _P(MangoError)]),
+    "mango_price_american_batch_adjoint": (C.c_int32, [_P(MangoPricingParams), C.c_uint64, _P(MangoGridConfig),
+                                                       _P(MangoAdjointGreeks), _P(C.c_int32), _P(C.c_uint64),
+                                                       _P(MangoError)]),
+    "mango_price_american_adjoint_device": (C.c_int32, [C.c_void_p, C.c_uint64, _P(MangoGridConfig), C.c_void_p,
+                                                        C.c_void_p, C.c_void_p, _P(MangoError)]),
     "mango_plan_grid_sizes": (C.c_int32, [C.c_void_p, C.c_uint64, _P(MangoGridConfig), _P(C.c_uint64),
                                           _P(C.c_uint64), _P(MangoError)]),
     "mango_tuning_set": (C.c_int32, [_P(MangoVariantRule), C.c_uint64, _P(MangoError)]),

@@ -1,0 +1,125 @@
+# SPDX-License-Identifier: MIT
+"""GPU tests of the reverse-mode adjoint kernel against the oracle's adjoint
+(CPU restatement; itself pinned by finite differences of the frozen-set map
+and by Black-Scholes limits in test_oracle_adjoint.py)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import mango_option as M
+from mango_option import _abi, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+PRICE_ABS = 1e-9
+GREEK_REL = 1e-7
+SENS_REL = 1e-7
+SENS_ABS = 1e-9
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _fixed_cfg():
+    acc = M.GridAccuracyParams(min_spatial_points=101, max_spatial_points=101)
+    return M.grid_config(_abi.MANGO_GRID_MODE_ACCURACY_FIXED_TIME, acc, n_time=498)
+
+
+def _uneven_cfg():
+    acc = M.GridAccuracyParams(tol=1e-3, min_spatial_points=51, max_spatial_points=401, max_time_steps=2000)
+    return M.grid_config(_abi.MANGO_GRID_MODE_ACCURACY, acc, min_time_steps=100)
+
+
+def _device_adjoint(opts, cfg):
+    import torch
+    lib = _abi.load_library()
+    n = len(opts)
+    d_opts = torch.from_numpy(opts.records().view(np.uint8)).cuda()
+    d_out = torch.full((n, 8), float("nan"), dtype=torch.float64, device="cuda")
+    d_st = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+    err = _abi.MangoError()
+    rc = lib.mango_price_american_adjoint_device(d_opts.data_ptr(), n, C.byref(cfg), d_out.data_ptr(),
+                                                 d_st.data_ptr(), C.c_void_p(torch.cuda.current_stream().cuda_stream),
+                                                 C.byref(err))
+    assert rc == 0, err.message.decode()
+    torch.cuda.synchronize()
+    return d_out.cpu().numpy(), d_st.cpu().numpy()
+
+
+def _check(out, st, ref, ref_st, label):
+    assert np.array_equal(st, ref_st), label
+    ok = ref_st == 0
+    assert np.max(np.abs(out[ok, 0] - ref[ok, 0]), initial=0.0) <= PRICE_ABS, label
+    d = np.abs(out[ok, 1] - ref[ok, 1])
+    assert (d <= GREEK_REL * np.abs(ref[ok, 1]) + 1e-10).all(), (label, "delta", d.max())
+    for col, name in ((2, "dV/dsigma"), (3, "dV/dr"), (4, "dV/dq"), (5, "dV/dK")):
+        a, b = out[ok, col + 2], ref[ok, col]
+        d = np.abs(a - b)
+        bad = d > SENS_REL * np.abs(b) + SENS_ABS
+        assert not bad.any(), (label, name, np.where(bad)[0][:5], a[bad][:5], b[bad][:5])
+
+
+def test_adjoint_fixed_grid_matches_oracle():
+    _say("test_adjoint_fixed_grid_matches_oracle: 64 puts + 64 chain options, 101x498")
+    puts = W.c1_puts(64, seed=5)
+    chains, _ = W.c2_chains(1, 64, seed=1)
+    opts = W.OptionBatch(*(np.concatenate([getattr(puts, f), getattr(chains, f)]) for f in
+                           ("spot", "strike", "maturity", "volatility", "rate", "dividend_yield", "option_type")))
+    cfg = _fixed_cfg()
+    out, st = _device_adjoint(opts, cfg)
+    ref, ref_st = O.adjoint_batch(opts, cfg, n_threads=8)
+    _check(out, st, ref, ref_st, "fixed")
+
+
+def test_adjoint_forward_equals_solve():
+    _say("test_adjoint_forward_equals_solve: 256 puts, value/delta/gamma/theta vs the solve kernel")
+    import torch
+    opts = W.c1_puts(256, seed=7)
+    cfg = _fixed_cfg()
+    out, st = _device_adjoint(opts, cfg)
+    lib = _abi.load_library()
+    d_opts = torch.from_numpy(opts.records().view(np.uint8)).cuda()
+    d_g = torch.empty((256, 4), dtype=torch.float64, device="cuda")
+    d_s = torch.empty(256, dtype=torch.int32, device="cuda")
+    err = _abi.MangoError()
+    assert lib.mango_price_american_batch_device(d_opts.data_ptr(), 256, C.byref(cfg), d_g.data_ptr(), d_s.data_ptr(),
+                                                 C.c_void_p(torch.cuda.current_stream().cuda_stream),
+                                                 C.byref(err)) == 0
+    torch.cuda.synchronize()
+    g = d_g.cpu().numpy()
+    assert (st == 0).all() and (d_s.cpu().numpy() == 0).all()
+    np.testing.assert_allclose(out[:, :4], g, rtol=1e-12, atol=1e-12)
+
+
+def test_adjoint_uneven_grids_match_oracle():
+    _say("test_adjoint_uneven_grids_match_oracle: 96 options, 51-401 points x 100-2000 steps")
+    opts = W.c3_uneven(96, seed=12)
+    cfg = _uneven_cfg()
+    out, st = _device_adjoint(opts, cfg)
+    ref, ref_st = O.adjoint_batch(opts, cfg, n_threads=8)
+    _check(out, st, ref, ref_st, "uneven")
+
+
+def test_adjoint_host_entry_and_invalid_slots():
+    _say("test_adjoint_host_entry_and_invalid_slots")
+    opts = W.c1_puts(16, seed=9)
+    opts.volatility[3] = -0.2
+    arr, n = O.c_params(opts)
+    cfg = _fixed_cfg()
+    out = (_abi.MangoAdjointGreeks * n)()
+    st = (C.c_int32 * n)()
+    failed = C.c_uint64()
+    err = _abi.MangoError()
+    rc = _abi.load_library().mango_price_american_batch_adjoint(arr, n, C.byref(cfg), out, st, C.byref(failed),
+                                                                C.byref(err))
+    assert rc == 0, err.message.decode()
+    h = np.frombuffer(out, dtype=np.float64).reshape(n, 8)
+    s = np.frombuffer(st, dtype=np.int32)
+    assert failed.value == 1 and s[3] == _abi.MANGO_ERR_VALIDATION and np.isnan(h[3]).all()
+    d, ds = _device_adjoint(opts, cfg)
+    ok = s == 0
+    assert np.array_equal(ds[ok], s[ok])
+    assert np.array_equal(d[ok], h[ok])
