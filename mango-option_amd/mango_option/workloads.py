@@ -139,3 +139,14 @@ def replicate(batch: OptionBatch, n: int) -> OptionBatch:
     """Tile a batch to n options (throughput variants of a configuration)."""
     idx = np.arange(n) % len(batch)
     return batch.take(idx)
+
+
+def c5_puts(n: int = 16384, seed: int = 5) -> OptionBatch:
+    """Adjoint configuration: American puts drawn like the plain configuration."""
+    return c1_puts(n, seed=seed)
+
+
+def hidden_sigmas(n: int, seed: int = 6) -> np.ndarray:
+    """Hidden volatilities sigma* ~ U[0.1, 0.6] for the calibration targets."""
+    g = Xoshiro256ss(seed)
+    return np.array([0.1 + 0.5 * g.uniform() for _ in range(n)])
