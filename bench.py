@@ -16,6 +16,9 @@ Also measured in the same run:
 - batched form: 65,536 options of uneven grid sizes (51-401 points x 100-2000
   steps) through the size-binned persistent work queues replayed from a
   hipGraph (mango_price_american_queue_device);
+- adjoint: 16,384 puts at 101x498, price plus dV/dsigma, dV/dr, dV/dq, dV/dK
+  by the reverse-mode kernel (mango_price_american_adjoint_device), with the
+  plain forward of the same options timed beside it;
 - cpu_baseline: the oracle (CPU restatement of the reference's OpenMP batch
   path) on the host cores, which also checks a sample of the GPU results.
 """
@@ -206,7 +209,41 @@ def run_batched(g: Gpu, args):
             "_cfg": cfg}
 
 
-def cpu_baseline(chains, batched, seconds: float):
+def run_adjoint(g: Gpu, args):
+    """Adjoint configuration: 16,384 puts (seed 5) at 101x498, forward plus
+    reverse sweep with sqrt(N) checkpoints in one call; the plain forward of
+    the same options timed beside it for the cost ratio."""
+    opts = g.W.c5_puts(args.adjoint_options, seed=5)
+    n = len(opts)
+    cfg = chain_grid_config(g.M)
+    d_opts, _, d_st = g.upload(opts)
+    d_adj = g.torch.empty((n, 8), dtype=g.torch.float64, device=g.dev)
+    d_fwd = g.torch.empty((n, 4), dtype=g.torch.float64, device=g.dev)
+
+    def adj_step():
+        g.check(g.lib.mango_price_american_adjoint_device(d_opts.data_ptr(), n, C.byref(cfg), d_adj.data_ptr(),
+                                                          d_st.data_ptr(), g.sp, C.byref(g.err)))
+
+    def fwd_step():
+        g.check(g.lib.mango_price_american_batch_device(d_opts.data_ptr(), n, C.byref(cfg), d_fwd.data_ptr(),
+                                                        d_st.data_ptr(), g.sp, C.byref(g.err)))
+
+    el_f, _, _ = g.timed(fwd_step, args.adjoint_steps, 1, kernel_timing=False)
+    el_a, _, _ = g.timed(adj_step, args.adjoint_steps, 1, kernel_timing=False)
+    st = d_st.cpu().numpy()
+    if (st != 0).any():
+        raise RuntimeError(f"{int((st != 0).sum())} adjoint options failed")
+    ms_a = 1e3 * el_a / args.adjoint_steps
+    ms_f = 1e3 * el_f / args.adjoint_steps
+    return {"workload": f"{n} puts of the adjoint configuration (seed 5), 101x498, forward + reverse sweep with "
+                        f"sqrt(N) checkpoints: value, delta, gamma, theta, dV/dsigma, dV/dr, dV/dq, dV/dK",
+            "options": n, "options_per_s": n / (ms_a * 1e-3), "sensitivities_per_s": 4 * n / (ms_a * 1e-3),
+            "ms_per_step": ms_a, "forward_ms_per_step": ms_f, "cost_vs_forward": ms_a / ms_f,
+            "_batch": opts.take(np.arange(args.adjoint_check)), "_out": d_adj[:args.adjoint_check].cpu().numpy(),
+            "_cfg": cfg}
+
+
+def cpu_baseline(chains, batched, seconds: float, adjoint=None):
     """The oracle (CPU restatement of the reference) pricing whole chains the
     reference's way: one solve_batch(group, use_shared_grid=true, grid) per
     (chain, type) with the chain's union grid; OpenMP across the regular
@@ -250,6 +287,11 @@ def cpu_baseline(chains, batched, seconds: float):
         gr, _, _, _ = O.price_batch(batched["_batch"], False, batched["_cfg"], use_normalized=False,
                                     n_threads=threads)
         out["parity_max_abs_price_err_batched"] = float(np.max(np.abs(gr[:, 0] - batched["_greeks"][:, 0])))
+    if adjoint is not None:
+        ref, _ = O.adjoint_batch(adjoint["_batch"], adjoint["_cfg"], n_threads=threads)
+        sens = adjoint["_out"][:, 4:8]
+        out["parity_max_rel_sensitivity_err_adjoint"] = float(
+            np.max(np.abs(sens - ref[:, 2:6]) / np.maximum(np.abs(ref[:, 2:6]), 1e-8)))
     return out
 
 
@@ -263,6 +305,9 @@ def main():
     ap.add_argument("--batched-options", type=int, default=65536)
     ap.add_argument("--batched-steps", type=int, default=3)
     ap.add_argument("--batched-check", type=int, default=256, help="batched options checked against the oracle")
+    ap.add_argument("--adjoint-options", type=int, default=16384)
+    ap.add_argument("--adjoint-steps", type=int, default=3)
+    ap.add_argument("--adjoint-check", type=int, default=128, help="adjoint options checked against the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", default="", help="force lanes,nodes_per_lane of the solve kernel at 101 points")
@@ -276,6 +321,7 @@ def main():
     chains = run_chains(g, args)
     plain = run_plain(g, args) if args.plain_options > 0 else None
     batched = run_batched(g, args) if args.batched_options > 0 else None
+    adjoint = run_adjoint(g, args) if args.adjoint_options > 0 else None
 
     workload = "chains_4096_101x498"
     flops_launch = chains["pde_solves"] * float(flops_per_option(N_SPACE, N_TIME))
@@ -306,7 +352,9 @@ def main():
         line["plain"] = {k: v for k, v in plain.items() if not k.startswith("_")}
     if batched is not None:
         line["batched"] = {k: v for k, v in batched.items() if not k.startswith("_")}
-    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(chains, batched, args.cpu_seconds)
+    if adjoint is not None:
+        line["adjoint"] = {k: v for k, v in adjoint.items() if not k.startswith("_")}
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(chains, batched, args.cpu_seconds, adjoint)
     print(json.dumps(line), flush=True)
 
 

@@ -150,3 +150,50 @@ def hidden_sigmas(n: int, seed: int = 6) -> np.ndarray:
     """Hidden volatilities sigma* ~ U[0.1, 0.6] for the calibration targets."""
     g = Xoshiro256ss(seed)
     return np.array([0.1 + 0.5 * g.uniform() for _ in range(n)])
+
+
+# ---- price-table configuration ----
+
+C4_K_REF = 100.0
+C4_DIVIDEND = 0.02
+
+
+def sqrt_uniform_grid(lo: float, hi: float, n: int) -> np.ndarray:
+    """n points uniform in sqrt-space on [lo, hi] (denser near lo), as the
+    reference's maturity-axis estimator lays them out
+    (src/option/table/bspline/bspline_builder.hpp:219-228)."""
+    s = math.sqrt(lo) + (math.sqrt(hi) - math.sqrt(lo)) * np.arange(n, dtype=np.float64) / (n - 1)
+    return np.array([v * v for v in s])
+
+
+def c4_axes():
+    """Price-table axes: 50 log-moneyness points on [-0.6, 0.4], 30 maturities
+    sqrt-uniform on [0.01, 2], 20 volatilities on [0.05, 0.8], 10 rates on
+    [0, 0.1] (put, K_ref = 100, dividend yield 0.02)."""
+    return (np.linspace(-0.6, 0.4, 50), sqrt_uniform_grid(0.01, 2.0, 30), np.linspace(0.05, 0.80, 20),
+            np.linspace(0.0, 0.10, 10))
+
+
+def splitmix64_uniform(seed: int, n: int, start: int = 0) -> np.ndarray:
+    """Counter-based uniforms: u_k = (splitmix64 output k >> 11) * 2**-53 for
+    k = start .. start+n-1 of the stream seeded with `seed` (vectorised; used
+    where millions of draws are needed)."""
+    k = np.arange(start + 1, start + n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def c4_query_points(n: int = 1 << 20, seed: int = 4):
+    """Interpolated-IV queries inside the table axes: columns (log-moneyness,
+    maturity, sigma, rate) uniform on each axis range and a price noise factor
+    1 + U[-0.01, 0.01]; draw k of query i is stream element 5 i + k."""
+    lm, tau, sig, rate = c4_axes()
+    u = splitmix64_uniform(seed, 5 * n).reshape(n, 5)
+    pts = np.empty((n, 4))
+    for a, g in enumerate((lm, tau, sig, rate)):
+        pts[:, a] = g[0] + (g[-1] - g[0]) * u[:, a]
+    return pts, 1.0 + 0.02 * (u[:, 4] - 0.5)

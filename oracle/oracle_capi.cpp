@@ -5,6 +5,7 @@
 // baseline can call it. Uses the struct layouts of include/mango_amd.h.
 #include "mango_oracle.hpp"
 #include "mango_adjoint.hpp"
+#include "mango_table.hpp"
 
 #include "../include/mango_amd.h"
 
@@ -269,6 +270,101 @@ uint64_t oracle_adjoint_batch(const MangoPricingParams* p, uint64_t n, const Man
         }
     }
     return failed;
+}
+
+// ---- price table / interpolated IV (mango_table.hpp) ----
+static std::array<std::vector<double>, 4> axes4(const double* a0, uint64_t n0, const double* a1, uint64_t n1,
+                                                const double* a2, uint64_t n2, const double* a3, uint64_t n3) {
+    return {std::vector<double>(a0, a0 + n0), std::vector<double>(a1, a1 + n1), std::vector<double>(a2, a2 + n2),
+            std::vector<double>(a3, a3 + n3)};
+}
+
+// PriceTableBuilderND<4>::build (explicit grid when spec != null, else the
+// accuracy path). out_coeffs: n0 n1 n2 n3 fitted EEP coefficients;
+// out_tensor (nullable): the normalized American prices before the EEP
+// transform. Returns 0 on success.
+int32_t oracle_table_build(const double* lm, uint64_t nm, const double* tau, uint64_t nt, const double* sig,
+                           uint64_t ns, const double* rate, uint64_t nr, double K_ref, double q, int32_t type,
+                           const MangoGridSpec* spec, uint64_t n_time, const MangoGridAccuracy* acc,
+                           double* out_coeffs, double* out_tensor, uint64_t* out_pde_solves, int32_t* out_fallback,
+                           double* out_max_residual, int32_t n_threads) {
+    const auto axes = axes4(lm, nm, tau, nt, sig, ns, rate, nr);
+    std::optional<PDEGridConfig> eg;
+    if (spec) eg = PDEGridConfig{to_spec(*spec), n_time};
+    PriceTable t;
+    TableBuildInfo info;
+    const bool ok = build_price_table(axes, K_ref, q, type == MANGO_PUT ? OptionType::PUT : OptionType::CALL, eg,
+                                      to_acc(acc), t, info, n_threads > 0 ? n_threads : 1);
+    if (out_pde_solves) *out_pde_solves = info.pde_solves;
+    if (out_fallback) *out_fallback = info.fallback ? 1 : 0;
+    if (out_max_residual) *out_max_residual = info.max_residual;
+    if (!ok) return 1;
+    std::copy(t.spline.coeffs.begin(), t.spline.coeffs.end(), out_coeffs);
+    if (out_tensor) std::copy(info.tensor.begin(), info.tensor.end(), out_tensor);
+    return 0;
+}
+
+double oracle_eep_floor(double x) { return eep_floor(x); }
+
+// Separable 4-D fit in place (values -> coefficients); returns 0 on success.
+int32_t oracle_bspline_fit4(const double* a0, uint64_t n0, const double* a1, uint64_t n1, const double* a2,
+                            uint64_t n2, const double* a3, uint64_t n3, double* values, double* out_max_residual) {
+    const auto axes = axes4(a0, n0, a1, n1, a2, n2, a3, n3);
+    std::vector<double> v(values, values + n0 * n1 * n2 * n3);
+    double r = 0.0;
+    if (!fit_separable4(axes, v, 1e-6, &r)) return 1;
+    std::copy(v.begin(), v.end(), values);
+    if (out_max_residual) *out_max_residual = r;
+    return 0;
+}
+
+// B-spline value (partial_axis < 0) or first partial at m points q[m][4].
+void oracle_bspline_eval4(const double* a0, uint64_t n0, const double* a1, uint64_t n1, const double* a2,
+                          uint64_t n2, const double* a3, uint64_t n3, const double* coeffs, const double* q,
+                          uint64_t m, int32_t partial_axis, double* out) {
+    BSpline4 b;
+    b.set(axes4(a0, n0, a1, n1, a2, n2, a3, n3), std::vector<double>(coeffs, coeffs + n0 * n1 * n2 * n3));
+    for (uint64_t i = 0; i < m; ++i) out[i] = b.eval(q + 4 * i, partial_axis);
+}
+
+// Interpolated IV for n queries (q_rows[i] = S, K, tau, r, q, market price).
+void oracle_iv_batch(const double* a0, uint64_t n0, const double* a1, uint64_t n1, const double* a2, uint64_t n2,
+                     const double* a3, uint64_t n3, const double* coeffs, double K_ref, double q, int32_t type,
+                     const double* q_rows, const int32_t* q_types, uint64_t n, const double* cfg4, double* out_iv,
+                     uint64_t* out_iters, double* out_err, int32_t* out_status, int32_t n_threads) {
+    PriceTable t;
+    t.spline.set(axes4(a0, n0, a1, n1, a2, n2, a3, n3), std::vector<double>(coeffs, coeffs + n0 * n1 * n2 * n3));
+    t.K_ref = K_ref;
+    t.q = q;
+    t.type = type == MANGO_PUT ? OptionType::PUT : OptionType::CALL;
+    IVConfig cfg;
+    if (cfg4) { cfg.max_iter = static_cast<size_t>(cfg4[0]); cfg.tolerance = cfg4[1]; cfg.sigma_min = cfg4[2]; cfg.sigma_max = cfg4[3]; cfg.vega_threshold = cfg4[4]; }
+    #pragma omp parallel for schedule(dynamic, 64) num_threads(n_threads > 0 ? n_threads : 1)
+    for (uint64_t i = 0; i < n; ++i) {
+        const double* r = q_rows + 6 * i;
+        const IVOut o = solve_iv(t, r[0], r[1], r[2], r[3], r[4], q_types[i] == MANGO_PUT ? OptionType::PUT : OptionType::CALL,
+                                 r[5], cfg);
+        out_iv[i] = o.status == 0 ? o.iv : kNaN;
+        out_iters[i] = o.iterations;
+        out_err[i] = o.final_error;
+        out_status[i] = o.status;
+    }
+}
+
+// Table price and vega at m points (rows S, K, tau, sigma, r).
+void oracle_table_price(const double* a0, uint64_t n0, const double* a1, uint64_t n1, const double* a2, uint64_t n2,
+                        const double* a3, uint64_t n3, const double* coeffs, double K_ref, double q, int32_t type,
+                        const double* rows, uint64_t m, double* out_price, double* out_vega) {
+    PriceTable t;
+    t.spline.set(axes4(a0, n0, a1, n1, a2, n2, a3, n3), std::vector<double>(coeffs, coeffs + n0 * n1 * n2 * n3));
+    t.K_ref = K_ref;
+    t.q = q;
+    t.type = type == MANGO_PUT ? OptionType::PUT : OptionType::CALL;
+    for (uint64_t i = 0; i < m; ++i) {
+        const double* r = rows + 5 * i;
+        out_price[i] = t.price(r[0], r[1], r[2], r[3], r[4]);
+        if (out_vega) out_vega[i] = t.vega(r[0], r[1], r[2], r[3], r[4]);
+    }
 }
 
 int32_t oracle_thomas(const double* lo, const double* dg, const double* up, const double* rhs,

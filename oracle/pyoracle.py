@@ -54,6 +54,16 @@ def load() -> C.CDLL:
                                             C.c_uint64, P(MangoAmericanGreeks)]),
         "oracle_adjoint_batch": (C.c_uint64, [P(MangoPricingParams), C.c_uint64, P(MangoGridConfig), _dp,
                                               P(C.c_int32), C.c_int32]),
+        "oracle_table_build": (C.c_int32, [_dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64,
+                                           C.c_double, C.c_double, C.c_int32, P(MangoGridSpec), C.c_uint64,
+                                           P(MangoGridAccuracy), _dp, _dp, P(C.c_uint64), P(C.c_int32), _dp,
+                                           C.c_int32]),
+        "oracle_eep_floor": (C.c_double, [C.c_double]),
+        "oracle_bspline_fit4": (C.c_int32, [_dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, _dp]),
+        "oracle_bspline_eval4": (None, [_dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, _dp, C.c_uint64, C.c_int32, _dp]),
+        "oracle_iv_batch": (None, [_dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_double, C.c_double, C.c_int32, _dp, P(C.c_int32), C.c_uint64,
+                                   _dp, _dp, P(C.c_uint64), _dp, P(C.c_int32), C.c_int32]),
+        "oracle_table_price": (None, [_dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_double, C.c_double, C.c_int32, _dp, C.c_uint64, _dp, _dp]),
         "oracle_thomas": (C.c_int32, [_dp, _dp, _dp, _dp, _dp, C.c_uint64]),
         "oracle_thomas_projected": (C.c_int32, [_dp, _dp, _dp, _dp, _dp, _dp, C.c_uint64]),
         "oracle_spline_eval": (C.c_int32, [_dp, _dp, C.c_uint64, _dp, C.c_uint64, _dp, _dp]),
@@ -206,3 +216,89 @@ def bs_price(S, K, tau, sigma, r, q, put=True) -> float:
 
 def bs_vega(S, K, tau, sigma, r, q) -> float:
     return load().oracle_bs_vega(S, K, tau, sigma, r, q)
+
+
+# ---- price table / interpolated IV ----
+
+def _axes_args(axes):
+    out = []
+    for a in axes:
+        out += [_p(a), len(a)]
+    return out
+
+
+class Table:
+    """A fitted 4-D EEP table: axes (log-moneyness, tau, sigma, rate),
+    coefficients [n0, n1, n2, n3], K_ref, dividend yield and option type."""
+
+    def __init__(self, axes, coeffs, K_ref, q, put=True, tensor=None, pde_solves=0, fallback=False,
+                 max_residual=0.0):
+        self.axes = [_a(a) for a in axes]
+        self.coeffs = _a(coeffs).reshape([len(a) for a in self.axes])
+        self.K_ref, self.q, self.put = float(K_ref), float(q), bool(put)
+        self.tensor, self.pde_solves, self.fallback, self.max_residual = tensor, pde_solves, fallback, max_residual
+
+    def price(self, rows):
+        """rows [m, 5] = S, K, tau, sigma, r -> (price, vega)."""
+        rows = _a(rows).reshape(-1, 5)
+        m = rows.shape[0]
+        pr, vg = np.empty(m), np.empty(m)
+        load().oracle_table_price(*_axes_args(self.axes), _p(self.coeffs), self.K_ref, self.q, 1 if self.put else 0,
+                                  _p(rows), m, _p(pr), _p(vg))
+        return pr, vg
+
+    def iv(self, rows, put=None, cfg=None, n_threads=1):
+        """rows [n, 6] = S, K, tau, r, q, market -> (iv, iterations, final_error,
+        status); cfg = (max_iter, tolerance, sigma_min, sigma_max, vega_threshold)."""
+        rows = _a(rows).reshape(-1, 6)
+        n = rows.shape[0]
+        types = np.full(n, 1 if (self.put if put is None else put) else 0, dtype=np.int32)
+        iv, err = np.empty(n), np.empty(n)
+        it = np.empty(n, dtype=np.uint64)
+        st = np.empty(n, dtype=np.int32)
+        c5 = _a(cfg) if cfg is not None else None
+        load().oracle_iv_batch(*_axes_args(self.axes), _p(self.coeffs), self.K_ref, self.q, 1 if self.put else 0,
+                               _p(rows), types.ctypes.data_as(C.POINTER(C.c_int32)), n,
+                               _p(c5) if c5 is not None else None, _p(iv),
+                               it.ctypes.data_as(C.POINTER(C.c_uint64)), _p(err),
+                               st.ctypes.data_as(C.POINTER(C.c_int32)), n_threads)
+        return iv, it, err, st
+
+
+def table_build(axes, K_ref, q, put=True, spec: MangoGridSpec | None = None, n_time=0,
+                acc: MangoGridAccuracy | None = None, n_threads=1) -> Table:
+    axes = [_a(a) for a in axes]
+    shape = [len(a) for a in axes]
+    size = int(np.prod(shape))
+    coeffs, tensor = np.empty(size), np.empty(size)
+    solves, fb, res = C.c_uint64(), C.c_int32(), C.c_double()
+    rc = load().oracle_table_build(*_axes_args(axes), float(K_ref), float(q), 1 if put else 0,
+                                   C.byref(spec) if spec is not None else None, n_time,
+                                   C.byref(acc) if acc is not None else None, _p(coeffs), _p(tensor),
+                                   C.byref(solves), C.byref(fb), C.byref(res), n_threads)
+    if rc != 0:
+        raise RuntimeError(f"oracle_table_build failed ({rc}): pde_solves={solves.value} fallback={fb.value} "
+                           f"max_residual={res.value}")
+    return Table(axes, coeffs, K_ref, q, put, tensor.reshape(shape), solves.value, bool(fb.value), res.value)
+
+
+def bspline_fit4(axes, values):
+    axes = [_a(a) for a in axes]
+    v = _a(values).copy()
+    res = C.c_double()
+    if load().oracle_bspline_fit4(*_axes_args(axes), _p(v), C.byref(res)) != 0:
+        raise RuntimeError("oracle_bspline_fit4 failed")
+    return v.reshape([len(a) for a in axes]), res.value
+
+
+def bspline_eval4(axes, coeffs, points, partial_axis=-1):
+    axes = [_a(a) for a in axes]
+    c = _a(coeffs)
+    pts = _a(points).reshape(-1, 4)
+    out = np.empty(pts.shape[0])
+    load().oracle_bspline_eval4(*_axes_args(axes), _p(c), _p(pts), pts.shape[0], int(partial_axis), _p(out))
+    return out
+
+
+def eep_floor(x: float) -> float:
+    return load().oracle_eep_floor(float(x))
