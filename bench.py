@@ -19,6 +19,9 @@ Also measured in the same run:
 - adjoint: 16,384 puts at 101x498, price plus dV/dsigma, dV/dr, dV/dq, dV/dK
   by the reverse-mode kernel (mango_price_american_adjoint_device), with the
   plain forward of the same options timed beside it;
+- price table: the 50 x 30 x 20 x 10 EEP B-spline table built on the GPU
+  from 200 snapshot PDE solves, and 1,048,576 interpolated-IV queries on it
+  (mango_price_table_build, mango_interp_iv_solve_device);
 - cpu_baseline: the oracle (CPU restatement of the reference's OpenMP batch
   path) on the host cores, which also checks a sample of the GPU results.
 """
@@ -243,7 +246,60 @@ def run_adjoint(g: Gpu, args):
             "_cfg": cfg}
 
 
-def cpu_baseline(chains, batched, seconds: float, adjoint=None):
+def run_table(g: Gpu, args):
+    """Price-table configuration: the 50 x 30 x 20 x 10 put table (K_ref 100,
+    q 0.02) from a 201-point sinh grid x 1000 steps (the builder's width
+    check replaces it by per-solve estimated grids), built on the GPU; then
+    1,048,576 interpolated-IV queries resident in HBM, prices drawn from the
+    table x (1 + U[-0.01, 0.01])."""
+    from mango_option.table import IV_QUERY_DTYPE, IV_RESULT_DTYPE
+    torch, M, W = g.torch, g.M, g.W
+    ax = W.c4_axes()
+    cfg = M.grid_config(g.abi.MANGO_GRID_MODE_EXPLICIT, spec=M.GridSpec(n_points=201), n_time=1000)
+    builder = M.PriceTableBuilder.from_vectors(*ax, W.C4_K_REF, cfg, M.OptionType.PUT, W.C4_DIVIDEND)
+    table = builder.build()  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.table_builds):
+        table = builder.build()
+    build_ms = 1e3 * (time.perf_counter() - t0) / args.table_builds
+    info = table.build_info
+    n = args.table_queries
+    pts, noise = W.c4_query_points(n, seed=4)
+    S = W.C4_K_REF * np.exp(pts[:, 0])
+    rows = torch.from_numpy(np.stack([S, np.full(n, W.C4_K_REF), pts[:, 1], pts[:, 2], pts[:, 3]], 1)).to(g.dev)
+    pr, _ = table.price_device(rows, want_vega=False)
+    rec = np.zeros(n, dtype=IV_QUERY_DTYPE)
+    rec["spot"], rec["strike"], rec["maturity"], rec["rate"] = S, W.C4_K_REF, pts[:, 1], pts[:, 3]
+    rec["dividend_yield"], rec["option_type"] = W.C4_DIVIDEND, 1
+    rec["market_price"] = pr.cpu().numpy() * noise
+    d_q = torch.from_numpy(rec.view(np.uint8).reshape(n, IV_QUERY_DTYPE.itemsize)).to(g.dev)
+    solver = table.make_iv_solver()
+    d_r = solver.solve_device(d_q)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.table_iv_steps):
+        solver.solve_device(d_q, d_r)
+    e1.record()
+    torch.cuda.synchronize()
+    iv_ms = e0.elapsed_time(e1) / args.table_iv_steps
+    res = d_r.cpu().numpy().reshape(-1).view(IV_RESULT_DTYPE)
+    st = res["status"]
+    k = args.table_check
+    _, coeffs, _ = table.export()
+    return {"workload": "price table 50 log-moneyness x 30 maturities x 20 vols x 10 rates (put, K_ref 100, "
+                        "q 0.02), 201-point grid x 1000 steps requested, 200 PDE solves; then 1,048,576 "
+                        "interpolated-IV queries (table prices x (1 + U[-1%, 1%]))",
+            "build_ms": build_ms, "builds_per_s": 1e3 / build_ms, "pde_solves": int(info.n_pde_solves),
+            "fallback_grid": bool(info.fallback_grid), "max_fit_residual": info.max_fit_residual,
+            "iv_queries": n, "iv_ms": iv_ms, "iv_queries_per_s": n / (iv_ms * 1e-3),
+            "iv_status_counts": {str(c): int(v) for c, v in enumerate(np.bincount(st, minlength=6))},
+            "iv_mean_brent_iterations": float(res["iterations"][st == 0].mean()),
+            "_rows": rec[:k], "_iv": res[:k], "_coeffs": coeffs}
+
+
+def cpu_baseline(chains, batched, seconds: float, adjoint=None, table=None):
     """The oracle (CPU restatement of the reference) pricing whole chains the
     reference's way: one solve_batch(group, use_shared_grid=true, grid) per
     (chain, type) with the chain's union grid; OpenMP across the regular
@@ -287,6 +343,24 @@ def cpu_baseline(chains, batched, seconds: float, adjoint=None):
         gr, _, _, _ = O.price_batch(batched["_batch"], False, batched["_cfg"], use_normalized=False,
                                     n_threads=threads)
         out["parity_max_abs_price_err_batched"] = float(np.max(np.abs(gr[:, 0] - batched["_greeks"][:, 0])))
+    if table is not None:
+        from mango_option import workloads as W
+        ax = W.c4_axes()
+        spec = M.GridSpec(n_points=201).to_c()
+        t0 = time.perf_counter()
+        ref = O.table_build(ax, W.C4_K_REF, W.C4_DIVIDEND, True, spec=spec, n_time=1000, n_threads=threads)
+        build_s = time.perf_counter() - t0
+        q = table["_rows"]
+        rows = np.stack([q["spot"], q["strike"], q["maturity"], q["rate"], q["dividend_yield"], q["market_price"]], 1)
+        t0 = time.perf_counter()
+        riv, _, _, rst = ref.iv(rows, n_threads=threads)
+        iv_s = time.perf_counter() - t0
+        giv, gst = table["_iv"]["implied_vol"], table["_iv"]["status"]
+        ok = (gst == 0) & (rst == 0)
+        out["table"] = {"build_ms": 1e3 * build_s, "iv_queries_per_s": len(rows) / iv_s, "cores": threads,
+                        "sample": f"one full table build and {len(rows)} of the IV queries on the oracle's table",
+                        "parity_iv_status_mismatches": int(np.sum(gst != rst)),
+                        "parity_max_abs_iv_err": float(np.max(np.abs(giv[ok] - riv[ok]), initial=0.0))}
     if adjoint is not None:
         ref, _ = O.adjoint_batch(adjoint["_batch"], adjoint["_cfg"], n_threads=threads)
         sens = adjoint["_out"][:, 4:8]
@@ -308,6 +382,11 @@ def main():
     ap.add_argument("--adjoint-options", type=int, default=16384)
     ap.add_argument("--adjoint-steps", type=int, default=3)
     ap.add_argument("--adjoint-check", type=int, default=128, help="adjoint options checked against the oracle")
+    ap.add_argument("--table-builds", type=int, default=5)
+    ap.add_argument("--table-queries", type=int, default=1 << 20)
+    ap.add_argument("--table-iv-steps", type=int, default=5)
+    ap.add_argument("--table-check", type=int, default=65536, help="IV queries checked against the oracle")
+    ap.add_argument("--no-table", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", default="", help="force lanes,nodes_per_lane of the solve kernel at 101 points")
@@ -322,6 +401,7 @@ def main():
     plain = run_plain(g, args) if args.plain_options > 0 else None
     batched = run_batched(g, args) if args.batched_options > 0 else None
     adjoint = run_adjoint(g, args) if args.adjoint_options > 0 else None
+    table = None if args.no_table else run_table(g, args)
 
     workload = "chains_4096_101x498"
     flops_launch = chains["pde_solves"] * float(flops_per_option(N_SPACE, N_TIME))
@@ -354,7 +434,9 @@ def main():
         line["batched"] = {k: v for k, v in batched.items() if not k.startswith("_")}
     if adjoint is not None:
         line["adjoint"] = {k: v for k, v in adjoint.items() if not k.startswith("_")}
-    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(chains, batched, args.cpu_seconds, adjoint)
+    if table is not None:
+        line["table"] = {k: v for k, v in table.items() if not k.startswith("_")}
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(chains, batched, args.cpu_seconds, adjoint, table)
     print(json.dumps(line), flush=True)
 
 

@@ -301,11 +301,212 @@ uint64_t mango_tuning_variants(MangoVariantRule* out, uint64_t cap);
 void mango_profile_enable(int32_t enable);
 MangoStatus mango_profile_read(double* solve_kernel_ms, uint64_t* solve_launches);
 
+/* ---- price table and interpolated IV: src/ffi/mango_c_api.h:46-189 -----
+ * The 4-D B-spline table of the early-exercise premium over (ln(S/K_ref),
+ * maturity, volatility, rate), built on the GPU (snapshot solves, spline
+ * extraction, EEP transform, separable collocation fit) and kept resident in
+ * device memory; queries (price, vega, interpolated IV) run on the GPU. */
+typedef struct {
+  double spot;
+  double strike;
+  double maturity;
+  double dividend_yield;
+  double market_price;
+  double rate_const;                    /* used iff n_tenor_points == 0 */
+  const MangoTenorPoint* tenor_points;
+  uint64_t n_tenor_points;
+  const MangoDividend* dividends;
+  uint64_t n_dividends;
+  MangoOptionType option_type;
+} MangoIvQuery;
+
+typedef struct {
+  double implied_vol;
+  uint64_t iterations;
+  double final_error;
+  double vega;
+  int32_t has_vega;
+  int32_t used_rate_approximation;
+} MangoIvSuccess;
+
+/* InterpolatedIVSolverConfig, applied verbatim (vega_threshold 0 disables
+ * the vega pre-check). mango_c_api.h:73-81 */
+typedef struct {
+  uint64_t max_iter;
+  double tolerance;
+  double sigma_min;
+  double sigma_max;
+  double vega_threshold;
+} MangoInterpSolverConfig;
+
+/* AdaptiveGridParams, MultiKRefConfig, DiscreteDividendConfig
+ * (mango_c_api.h:83-110). Accepted for layout compatibility; adaptive
+ * refinement and discrete-dividend (segmented) tables are not built on the
+ * GPU: a config carrying either returns MANGO_ERR_VALIDATION. */
+typedef struct {
+  double target_iv_error;
+  uint64_t max_iter;
+  uint64_t max_points_per_dim;
+  uint64_t min_moneyness_points;
+  uint64_t validation_samples;
+  double refinement_factor;
+  uint64_t lhs_seed;
+  double vega_floor;
+  double max_failure_rate;
+} MangoAdaptiveGridParams;
+
+typedef struct {
+  const double* K_refs;
+  uint64_t n_K_refs;
+  int32_t K_ref_count;
+  double K_ref_span;
+} MangoMultiKRef;
+
+typedef struct {
+  double maturity;
+  const MangoDividend* dividends;
+  uint64_t n_dividends;
+  MangoMultiKRef kref_config;
+} MangoDiscreteDividendConfig;
+
+/* IVSolverFactoryConfig, B-spline backend (mango_c_api.h:112-128):
+ * moneyness is S/K (converted to ln), spot is K_ref, maturity_grid the tau
+ * axis; the PDE grid is the builder's default GridAccuracyParams. */
+typedef struct {
+  MangoOptionType option_type;
+  double spot;
+  double dividend_yield;
+  const double* moneyness;
+  uint64_t n_moneyness;
+  const double* vol;
+  uint64_t n_vol;
+  const double* rate;
+  uint64_t n_rate;
+  const double* maturity_grid;
+  uint64_t n_maturity;
+  MangoInterpSolverConfig solver_config;
+  const MangoAdaptiveGridParams* adaptive;
+  const MangoDiscreteDividendConfig* discrete_dividends;
+} MangoIvFactoryConfig;
+
+typedef struct {
+  int32_t status;
+  MangoIvSuccess success;
+} MangoIvBatchSlot;
+
+typedef struct MangoInterpIvSolver MangoInterpIvSolver;  /* opaque */
+typedef struct MangoPriceTable MangoPriceTable;          /* opaque */
+
+/* make_price_table (src/option/price_table_factory.cpp:262-327, 836-848):
+ * PriceTableBuilder::from_vectors(ln moneyness, maturity, vol, rate, spot,
+ * GridAccuracyParams{}, type, dividend_yield) then build with the EEP
+ * transform. Replaces mango_c_api.h:170-171. */
+MangoStatus mango_make_price_table(const MangoIvFactoryConfig* cfg, MangoPriceTable** out, MangoError* err);
+/* AnyPriceTable::validate_pricing_params (price_table_factory.cpp:700-744).
+ * Replaces mango_c_api.h:172-173. */
+MangoStatus mango_price_table_validate(const MangoPriceTable* t, const MangoPricingParams* p, MangoError* err);
+/* Table price / vega at one point (NaN on null or unusable params;
+ * extrapolates by clamping to the axes). Replaces mango_c_api.h:174-175. */
+double mango_price_table_price(const MangoPriceTable* t, const MangoPricingParams* p);
+double mango_price_table_vega(const MangoPriceTable* t, const MangoPricingParams* p);
+MangoOptionType mango_price_table_option_type(const MangoPriceTable* t);
+double mango_price_table_dividend_yield(const MangoPriceTable* t);
+/* AnyPriceTable::make_iv_solver; cfg nullable (defaults). Replaces
+ * mango_c_api.h:186-188. */
+MangoStatus mango_price_table_make_iv_solver(const MangoPriceTable* t, const MangoInterpSolverConfig* cfg,
+                                             MangoInterpIvSolver** out, MangoError* err);
+void mango_price_table_free(MangoPriceTable* t);
+
+/* make_interpolated_iv_solver (price_table_factory.cpp:884-890): the table of
+ * mango_make_price_table with cfg->solver_config. Replaces
+ * mango_c_api.h:158-159. */
+MangoStatus mango_make_interp_iv_solver(const MangoIvFactoryConfig* cfg, MangoInterpIvSolver** out, MangoError* err);
+/* InterpolatedIVSolver::solve / solve_batch (interpolated_iv_solver.hpp:
+ * 431-556); status per query as map_iv_error (mango_c_api.cpp:124-141).
+ * Replace mango_c_api.h:160-166. Constant rates only: a query with tenor
+ * points is rejected with MANGO_ERR_VALIDATION. */
+MangoStatus mango_interp_iv_solve(const MangoInterpIvSolver* s, const MangoIvQuery* q, MangoIvSuccess* out,
+                                  MangoError* err);
+MangoStatus mango_interp_iv_solve_batch(const MangoInterpIvSolver* s, const MangoIvQuery* queries, uint64_t n,
+                                        MangoIvBatchSlot* out_slots, uint64_t* out_failed_count, MangoError* err);
+void mango_interp_iv_solver_free(MangoInterpIvSolver* s);
+
+/* ---- price table (additions) --------------------------------------------
+ * PriceTableBuilder::from_vectors(...) + build(axes, EEP transform)
+ * (src/option/table/bspline/bspline_builder.cpp:26-134, 544-594;
+ * price_table_factory.cpp:304-316) with the PDE grid chosen by `pde_grid`:
+ * MANGO_GRID_MODE_EXPLICIT is a PDEGridConfig (the builder checks it and
+ * falls back to an estimated grid when it is too wide or too coarse,
+ * bspline_builder.cpp:300-371), MANGO_GRID_MODE_ACCURACY the builder's
+ * GridAccuracyParams path (null: defaults). Axes are sorted and
+ * deduplicated; K_ref is the strike of the normalized solves. */
+typedef struct {
+  uint64_t n_pde_solves;
+  uint64_t failed_pde;
+  uint64_t repaired_full_slices;
+  uint64_t repaired_partial_points;
+  int32_t fallback_grid;         /* explicit grid replaced by an estimate */
+  int32_t reserved;
+  double max_fit_residual;
+  double build_ms;               /* wall time of the call */
+} MangoTableBuildInfo;
+MangoStatus mango_price_table_build(const double* log_moneyness, uint64_t n_m, const double* maturity,
+                                    uint64_t n_tau, const double* vol, uint64_t n_vol, const double* rate,
+                                    uint64_t n_rate, double K_ref, double dividend_yield, MangoOptionType type,
+                                    const MangoGridConfig* pde_grid, double max_failure_rate,
+                                    MangoPriceTable** out, MangoTableBuildInfo* info, MangoError* err);
+/* Axis sizes (n_m, n_tau, n_vol, n_rate) and reference strike. */
+MangoStatus mango_price_table_shape(const MangoPriceTable* t, uint64_t* dims, double* K_ref, MangoError* err);
+/* Copies to host memory (each nullable): the four axes back to back, the
+ * fitted coefficients [n_m][n_tau][n_vol][n_rate] and the extracted
+ * normalized American prices V/K_ref before the EEP transform. */
+MangoStatus mango_price_table_export(const MangoPriceTable* t, double* axes, double* coeffs, double* prices,
+                                     MangoError* err);
+/* Price and vega (vega nullable) at n points, host memory. */
+MangoStatus mango_price_table_price_batch(const MangoPriceTable* t, const MangoPricingParams* params, uint64_t n,
+                                          double* out_price, double* out_vega, MangoError* err);
+/* Device buffers, asynchronous on `stream`: rows d_points[i] = (S, K, tau,
+ * sigma, r), outputs d_price and d_vega (nullable). */
+MangoStatus mango_price_table_price_device(const MangoPriceTable* t, const double* d_points, uint64_t n,
+                                           double* d_price, double* d_vega, void* stream, MangoError* err);
+
+/* Interpolated IV over queries resident in device memory (56-byte records)
+ * into 24-byte results, asynchronous on `stream`. status as the slots of
+ * mango_interp_iv_solve_batch; implied_vol is NaN unless status is MANGO_OK. */
+typedef struct {
+  double spot, strike, maturity, rate, dividend_yield, market_price;
+  MangoOptionType option_type;
+  int32_t reserved;
+} MangoDeviceIvQuery;
+typedef struct {
+  double implied_vol;
+  double final_error;
+  int32_t iterations;
+  int32_t status;
+} MangoDeviceIvResult;
+MangoStatus mango_interp_iv_solve_device(const MangoInterpIvSolver* s, const MangoDeviceIvQuery* d_queries,
+                                         uint64_t n, MangoDeviceIvResult* d_out, void* stream, MangoError* err);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif
 
 #ifdef __cplusplus
+static_assert(sizeof(MangoIvQuery) == 88, "MangoIvQuery size");
+static_assert(offsetof(MangoIvQuery, market_price) == 32, "market_price off");
+static_assert(offsetof(MangoIvQuery, option_type) == 80, "iv option_type off");
+static_assert(sizeof(MangoIvSuccess) == 40, "MangoIvSuccess size");
+static_assert(sizeof(MangoInterpSolverConfig) == 40, "MangoInterpSolverConfig size");
+static_assert(sizeof(MangoAdaptiveGridParams) == 72, "MangoAdaptiveGridParams size");
+static_assert(sizeof(MangoMultiKRef) == 32, "MangoMultiKRef size");
+static_assert(sizeof(MangoDiscreteDividendConfig) == 56, "MangoDiscreteDividendConfig size");
+static_assert(sizeof(MangoIvFactoryConfig) == 144, "MangoIvFactoryConfig size");
+static_assert(offsetof(MangoIvFactoryConfig, solver_config) == 88, "fc solver_config off");
+static_assert(offsetof(MangoIvFactoryConfig, adaptive) == 128, "fc adaptive off");
+static_assert(sizeof(MangoIvBatchSlot) == 48, "MangoIvBatchSlot size");
+static_assert(sizeof(MangoTableBuildInfo) == 56, "MangoTableBuildInfo size");
+static_assert(sizeof(MangoDeviceIvQuery) == 56, "MangoDeviceIvQuery size");
+static_assert(sizeof(MangoDeviceIvResult) == 24, "MangoDeviceIvResult size");
 static_assert(sizeof(MangoPricingParams) == 88, "MangoPricingParams size");
 static_assert(offsetof(MangoPricingParams, rate_const) == 40, "rate_const off");
 static_assert(offsetof(MangoPricingParams, tenor_points) == 48, "tenor_points off");

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "../../include/mango_amd.h"
@@ -20,5 +21,16 @@ struct HostState {
 
 void set_err(MangoError* err, MangoStatus code, const char* msg);
 bool device_ready(MangoError* err);
+
+// MangoGridAccuracy (nullable: defaults) and MangoGridSpec conversions.
+Accuracy to_accuracy(const MangoGridAccuracy* a);
+GridDesc to_desc(const MangoGridSpec& s);
+// validate_pricing_params for the device path (constant rate, no discrete
+// dividends); *why names the first violation.
+MangoStatus validate_params(const MangoPricingParams& p, const char** why);
+// estimate_batch_pde_grid (grid_spec_types.cpp:120-177): union domain,
+// largest point and step counts of the per-option estimates.
+std::pair<GridDesc, int64_t> estimate_batch_grid_host(const MangoPricingParams* ps, size_t n, const Accuracy& acc,
+                                                      double* max_maturity);
 
 }  // namespace mango_amd

@@ -1,0 +1,495 @@
+// SPDX-License-Identifier: MIT
+//
+// gfx950 kernels of the price-table path:
+//   table_snapshot_kernel  the (sigma, r) solves of PriceTableBuilderND<4>::
+//                          solve_batch with maturity snapshots (src/option/
+//                          table/bspline/bspline_builder.cpp:280-371; Grid::
+//                          record, src/pde/core/grid.hpp:745-995)
+//   table_extract_kernel   extract_tensor (bspline_builder.cpp:395-480): a
+//                          natural spline of each snapshot evaluated at the
+//                          log-moneyness axis, the tau-repair of columns with
+//                          no snapshot (:802-861), and the EEP transform
+//                          (eep_decomposer.hpp:22-66, AnalyticalEEP,
+//                          european_option.hpp:24-66)
+//   fit_axis_kernel        one axis of the separable collocation fit
+//                          (bspline_nd_separable.hpp:304-384): every slice
+//                          solves with the axis' LU factors, computed once on
+//                          the host, and reports its residual |B c - f|
+//   table_price_kernel     EEPLayer<TransformLeaf> price and vega
+//                          (eep_layer.hpp:21-31, transform_leaf.hpp:28-46)
+//                          over BSplineND::eval / partial (bspline_nd.hpp:
+//                          133-340)
+//   interp_iv_kernel       InterpolatedIVSolver::solve (interpolated_iv_
+//                          solver.hpp:399-522) with Brent's method
+//                          (root_finding.hpp:145-296), one query per thread
+//
+// The snapshot solve reuses the TR-BDF2 march of the solve kernels (one
+// option per group of K lanes, M nodes per lane). The rest is one thread per
+// independent unit (a spline column, a fit slice, a query); the B-spline
+// coefficient tensor (2.4 MB at 50 x 30 x 20 x 10) stays resident in L2.
+#include "american_device.hpp"
+#include "bspline_basis.hpp"
+#include "price_table.hpp"
+
+namespace mango_amd {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Snapshot solves
+// ---------------------------------------------------------------------------
+template <int K, int M>
+__global__ __launch_bounds__(kSolveBlock) void table_snapshot_kernel(const PdeJob* __restrict__ jobs, int n_jobs,
+                                                                     const int32_t* __restrict__ steps,
+                                                                     const int32_t* __restrict__ n_snap, int max_snap,
+                                                                     int row, double* __restrict__ x_out,
+                                                                     double* __restrict__ snaps,
+                                                                     int32_t* __restrict__ status_out) {
+    constexpr int NMAX = K * M;
+    constexpr int GPB = kSolveBlock / K;
+    __shared__ double s_x[GPB][NMAX];
+    const int slot = threadIdx.x / K;
+    const int lane = threadIdx.x % K;
+    const int j = blockIdx.x * GPB + slot;
+    if (j >= n_jobs) return;  // the whole group leaves
+    const PdeJob job = jobs[j];
+    OptionPde<K, M> P;
+    P.init(job, lane, s_x[slot]);
+    const int n = P.n;
+    const int base = P.base;
+    const int64_t N = P.valid ? P.steps(job) : 0;
+    bool ok = P.valid && N > 0 && n <= row;
+    const double dt = ok ? job.maturity / static_cast<double>(N) : 0.0;
+    const int32_t* st = steps + static_cast<size_t>(j) * max_snap;
+    const int ns = n_snap[j];
+    if (ok) {
+        double* xr = x_out + static_cast<size_t>(j) * row;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            if (i < n) xr[i] = s_x[slot][i];
+        }
+    }
+    StageCache<M> c0, c1;
+    c0.invalidate();
+    c1.invalidate();
+    double u[M], mid[M];
+    P.initial(u);
+    int next = 0;
+    // the list is ascending and deduplicated: at most one row per state
+    auto record = [&](int64_t state) {
+        if (next < ns && st[next] == state) {
+            double* sr = snaps + (static_cast<size_t>(j) * max_snap + next) * row;
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                const int i = base + k;
+                if (i < n) sr[i] = u[k];
+            }
+            ++next;
+        }
+    };
+    if (ok) record(0);
+    for (int64_t s = 0; ok && s < N; ++s) {
+        ok = P.advance(s, dt, u, mid, c0, c1);
+        if (ok) record(s + 1);
+    }
+    if (lane == 0) status_out[j] = !P.valid ? kStatusValidation : (ok && next == ns ? kStatusOk : kStatusSolver);
+}
+
+// ---------------------------------------------------------------------------
+// European closed form and the EEP floor
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double norm_cdf(double x) { return 0.5 * erfc(-x * 0.70710678118654752440); }
+__device__ __forceinline__ double norm_pdf(double x) { return 0.3989422804014327 * exp(-0.5 * x * x); }
+
+__device__ __forceinline__ double european_price(double S, double K, double tau, double sigma, double r, double q,
+                                                 bool put) {
+    if (tau <= 0.0 || sigma <= 0.0) {
+        if (tau > 0.0) {
+            S = S * exp(-q * tau);
+            K = K * exp(-r * tau);
+        }
+        return put ? fmax(K - S, 0.0) : fmax(S - K, 0.0);
+    }
+    const double sst = sigma * sqrt(tau);
+    const double d1 = (log(S / K) + (r - q + 0.5 * sigma * sigma) * tau) / sst;
+    const double d2 = d1 - sigma * sqrt(tau);
+    const double eq = exp(-q * tau), er = exp(-r * tau);
+    if (put) return K * er * norm_cdf(-d2) - S * eq * norm_cdf(-d1);
+    return S * eq * norm_cdf(d1) - K * er * norm_cdf(d2);
+}
+
+__device__ __forceinline__ double european_vega(double S, double K, double tau, double sigma, double r, double q) {
+    if (tau <= 0.0 || sigma <= 0.0) return 0.0;
+    const double sst = sigma * sqrt(tau);
+    const double d1 = (log(S / K) + (r - q + 0.5 * sigma * sigma) * tau) / sst;
+    return S * exp(-q * tau) * sqrt(tau) * norm_pdf(d1);
+}
+
+// softplus with sharpness 100 minus its value at 0, floored at 0; the
+// identity once 100 x exceeds 500
+__device__ __forceinline__ double eep_floor(double x) {
+    if (100.0 * x > 500.0) return x;
+    const double softplus = log1p(exp(100.0 * x)) / 100.0;
+    const double bias = 0.69314718055994530942 / 100.0;
+    const double v = softplus - bias;
+    return v > 0.0 ? v : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Extraction + EEP
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void table_extract_kernel(ExtractArgs a) {
+    const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t n_slices = static_cast<int64_t>(a.n_sigma) * a.n_rate;
+    if (tid >= n_slices * a.n_tau) return;
+    const int s = static_cast<int>(tid / a.n_tau);
+    const int jc = static_cast<int>(tid % a.n_tau);
+    const int src = a.src_job[s];
+    const int ns = a.n_snap[src];
+    const int k = jc < ns ? jc : ns - 1;
+    const int n = static_cast<int>(a.n_space[src]);
+    const double* x = a.x + static_cast<size_t>(src) * a.row;
+    const double* y = a.snaps + (static_cast<size_t>(src) * a.max_snap + k) * a.row;
+    double* cp = a.spline_scratch + static_cast<size_t>(tid) * 2 * a.row;
+    double* c = cp + a.row;
+    const bool ok = k >= 0 && n >= 2 && spline_second(x, y, n, cp, c);
+    const int si = s / a.n_rate;
+    const int ri = s % a.n_rate;
+    const double tau = a.tau[jc], sigma = a.sigma[si], r = a.rate[ri];
+    const bool put = a.option_type == 1;
+    for (int i = 0; i < a.n_m; ++i) {
+        const size_t idx = ((static_cast<size_t>(i) * a.n_tau + jc) * a.n_sigma + si) * a.n_rate + ri;
+        const double v = ok ? spline_eval(x, y, c, n, a.lm[i]) : NAN;
+        if (a.raw) a.raw[idx] = v;
+        const double am = a.K_ref * v;
+        const double eu = european_price(exp(a.lm[i]) * a.K_ref, a.K_ref, tau, sigma, r, a.dividend_yield, put);
+        a.eep[idx] = eep_floor(am - eu);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Separable fit, one axis
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void fit_axis_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                                          int64_t total, int na, int64_t stride,
+                                                          const double* __restrict__ lu,
+                                                          const int32_t* __restrict__ piv,
+                                                          const double* __restrict__ B,
+                                                          unsigned long long* __restrict__ max_res) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t n_slices = total / na;
+    if (t >= n_slices) return;
+    const int64_t outer = t / stride;
+    const int64_t inner = t - outer * stride;
+    const double* f = src + outer * stride * na + inner;
+    double* y = dst + outer * stride * na + inner;
+    for (int i = 0; i < na; ++i) y[i * stride] = f[i * stride];
+    for (int k = 0; k < na; ++k) {
+        const int p = piv[k];
+        if (p != k) {
+            const double tmp = y[k * stride];
+            y[k * stride] = y[p * stride];
+            y[p * stride] = tmp;
+        }
+    }
+    for (int r = 0; r < na; ++r) {
+        double v = y[r * stride];
+        for (int j = 0; j < r; ++j) v -= lu[r * na + j] * y[j * stride];
+        y[r * stride] = v;
+    }
+    for (int r = na - 1; r >= 0; --r) {
+        double v = y[r * stride];
+        for (int j = r + 1; j < na; ++j) v -= lu[r * na + j] * y[j * stride];
+        v /= lu[r * na + r];
+        y[r * stride] = v;
+    }
+    double res = 0.0;
+    for (int i = 0; i < na; ++i) {
+        double bc = 0.0;
+        for (int j = 0; j < na; ++j) bc = fma(B[i * na + j], y[j * stride], bc);
+        const double d = fabs(bc - f[i * stride]);
+        res = res < d ? d : res;
+    }
+    if (!(res <= 0.0)) atomicMax(max_res, static_cast<unsigned long long>(__double_as_longlong(res)));
+}
+
+// ---------------------------------------------------------------------------
+// B-spline evaluation (clamped cubic knots)
+// ---------------------------------------------------------------------------
+struct AxisWeights {
+    int span;
+    double w[4];
+};
+
+// clamp_bspline_query (bspline_nd.hpp:40-62) then span and basis of one axis
+__device__ __forceinline__ AxisWeights axis_weights(const DeviceTable& T, int a, double x, bool deriv) {
+    const double* g = T.grid[a];
+    const int n = T.n[a];
+    if (x <= g[0]) x = g[0];
+    else if (x >= g[n - 1]) x = nextafter(g[n - 1], -INFINITY);
+    AxisWeights aw;
+    aw.span = find_span(T.knots[a], n, x);
+    cubic_basis(T.knots[a], n + 4, aw.span, x, aw.w, deriv);
+    return aw;
+}
+
+// Tensor-product sum, axis 3 innermost, in the order of BSplineND::eval.
+__device__ __forceinline__ double tensor_sum(const DeviceTable& T, const AxisWeights& a0, const AxisWeights& a1,
+                                             const AxisWeights& a2, const AxisWeights& a3) {
+    const int n0 = T.n[0], n1 = T.n[1], n2 = T.n[2], n3 = T.n[3];
+    const double* __restrict__ c = T.coeffs;
+    double s0 = 0.0;
+#pragma unroll
+    for (int o0 = 0; o0 < 4; ++o0) {
+        const int i0 = a0.span - o0;
+        if (static_cast<unsigned>(i0) >= static_cast<unsigned>(n0)) continue;
+        double s1 = 0.0;
+#pragma unroll
+        for (int o1 = 0; o1 < 4; ++o1) {
+            const int i1 = a1.span - o1;
+            if (static_cast<unsigned>(i1) >= static_cast<unsigned>(n1)) continue;
+            double s2 = 0.0;
+#pragma unroll
+            for (int o2 = 0; o2 < 4; ++o2) {
+                const int i2 = a2.span - o2;
+                if (static_cast<unsigned>(i2) >= static_cast<unsigned>(n2)) continue;
+                const size_t row = ((static_cast<size_t>(i0) * n1 + i1) * n2 + i2) * n3;
+                double s3 = 0.0;
+#pragma unroll
+                for (int o3 = 0; o3 < 4; ++o3) {
+                    const int i3 = a3.span - o3;
+                    if (static_cast<unsigned>(i3) >= static_cast<unsigned>(n3)) continue;
+                    s3 = fma(c[row + i3], a3.w[o3], s3);
+                }
+                s2 = fma(s3, a2.w[o2], s2);
+            }
+            s1 = fma(s2, a1.w[o1], s1);
+        }
+        s0 = fma(s1, a0.w[o0], s0);
+    }
+    return s0;
+}
+
+// A query point with its sigma-independent axes resolved once.
+struct TablePoint {
+    AxisWeights m, tau, r;
+    double S, K, tau_v, r_v;
+};
+
+__device__ __forceinline__ TablePoint table_point(const DeviceTable& T, double S, double K, double tau, double r) {
+    TablePoint p;
+    p.m = axis_weights(T, 0, log(S / K), false);
+    p.tau = axis_weights(T, 1, tau, false);
+    p.r = axis_weights(T, 3, r, false);
+    p.S = S; p.K = K; p.tau_v = tau; p.r_v = r;
+    return p;
+}
+
+__device__ __forceinline__ double point_price(const DeviceTable& T, const TablePoint& p, double sigma) {
+    const AxisWeights s = axis_weights(T, 2, sigma, false);
+    const double raw = tensor_sum(T, p.m, p.tau, s, p.r);
+    return (0.0 < raw ? raw : 0.0) * p.K / T.K_ref +
+           european_price(p.S, p.K, p.tau_v, sigma, p.r_v, T.dividend_yield, T.option_type == 1);
+}
+
+__device__ __forceinline__ double point_vega(const DeviceTable& T, const TablePoint& p, double sigma) {
+    const AxisWeights s = axis_weights(T, 2, sigma, false);
+    const double raw = tensor_sum(T, p.m, p.tau, s, p.r);
+    double leaf = 0.0;
+    if (!(raw <= 0.0)) {
+        const AxisWeights ds = axis_weights(T, 2, sigma, true);
+        leaf = tensor_sum(T, p.m, p.tau, ds, p.r) * p.K / T.K_ref;
+    }
+    return leaf + european_vega(p.S, p.K, p.tau_v, sigma, p.r_v, T.dividend_yield);
+}
+
+__global__ __launch_bounds__(kBlock) void table_price_kernel(DeviceTable T, const double* __restrict__ rows, int64_t n,
+                                                             double* __restrict__ price, double* __restrict__ vega) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* q = rows + 5 * i;
+    const TablePoint p = table_point(T, q[0], q[1], q[2], q[4]);
+    price[i] = point_price(T, p, q[3]);
+    if (vega) vega[i] = point_vega(T, p, q[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Interpolated implied volatility
+// ---------------------------------------------------------------------------
+constexpr int32_t kIvValidation = 1, kIvArbitrage = 2, kIvNoConvergence = 3, kIvBracketing = 4, kIvSolver = 5;
+
+__global__ __launch_bounds__(kBlock) void interp_iv_kernel(DeviceTable T, IvConfig cfg,
+                                                           const IvQueryRec* __restrict__ queries, int64_t n,
+                                                           IvResultRec* __restrict__ out) {
+    const int64_t qi = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (qi >= n) return;
+    const IvQueryRec q = queries[qi];
+    IvResultRec res{NAN, 0.0, 0, kIvValidation};
+    const double S = q.spot, K = q.strike, tau = q.maturity, r = q.rate, div = q.dividend_yield;
+    const double market = q.market_price;
+    const bool put = q.option_type == 1;
+    // validate_query then validate_iv_query, in the reference's order
+    int32_t st = 0;
+    if (q.option_type != T.option_type || fabs(div - T.dividend_yield) > 1e-10) st = kIvValidation;
+    else if (!(S > 0.0) || !isfinite(S) || !(K > 0.0) || !isfinite(K) || !(tau > 0.0) || !isfinite(tau))
+        st = kIvValidation;
+    else if (!isfinite(r) || div < 0.0 || !isfinite(div)) st = kIvArbitrage;
+    else if (!isfinite(market) || market <= 0.0) st = kIvValidation;
+    double intr = 0.0;
+    if (st == 0) {
+        intr = put ? fmax(K - S, 0.0) : fmax(S - K, 0.0);
+        const double upper = put ? K : S;
+        if (market < intr || market > upper) st = kIvArbitrage;
+    }
+    double smin = 0.0, smax = 0.0;
+    if (st == 0) {
+        // adaptive_bounds
+        const double tv = market - intr;
+        const double tvp = tv / market;
+        const double su = tvp > 0.5 ? 3.0 : (tvp > 0.2 ? 2.0 : 1.5);
+        const double g_lo = T.grid[2][0], g_hi = T.grid[2][T.n[2] - 1];
+        smin = fmax(cfg.sigma_min, g_lo);
+        smax = fmin(fmin(su, cfg.sigma_max), g_hi);
+        if (smin >= smax) { smin = g_lo; smax = g_hi; }
+        const double x = log(S / K);
+        auto inb = [&](double v) {
+            return x >= T.grid[0][0] && x <= T.grid[0][T.n[0] - 1] && tau >= T.grid[1][0] &&
+                   tau <= T.grid[1][T.n[1] - 1] && v >= g_lo && v <= g_hi && r >= T.grid[3][0] &&
+                   r <= T.grid[3][T.n[3] - 1];
+        };
+        if (!inb(smin) || !inb(smax)) st = kIvValidation;
+    }
+    if (st != 0) {
+        res.status = st;
+        out[qi] = res;
+        return;
+    }
+    // the pricing point of the objective: spot = (S / K) K
+    const double moneyness = S / K;
+    const TablePoint pv = table_point(T, S, K, tau, r);
+    if (cfg.vega_threshold > 0.0) {
+        double mv = 0.0;
+        const double probes[3] = {0.10, 0.25, 0.50};
+        for (int k = 0; k < 3; ++k) mv = fmax(mv, fabs(point_vega(T, pv, probes[k])));
+        if (mv < cfg.vega_threshold) {
+            res.status = kIvSolver;
+            res.final_error = mv;
+            out[qi] = res;
+            return;
+        }
+    }
+    const TablePoint pp = table_point(T, moneyness * K, K, tau, r);
+    auto f = [&](double s) { return point_price(T, pp, s) - market; };
+    // Brent's method
+    const double tol = cfg.tolerance;
+    double a = smin, b = smax;
+    double fa = f(a), fb = f(b);
+    if (!isfinite(fa) || !isfinite(fb)) {
+        res.status = kIvSolver;
+        res.final_error = NAN;
+        out[qi] = res;
+        return;
+    }
+    if (fa * fb > 0.0) {
+        res.status = kIvBracketing;
+        res.final_error = fmin(fabs(fa), fabs(fb));
+        out[qi] = res;
+        return;
+    }
+    if (fabs(fa) < tol) { res = IvResultRec{a, fabs(fa), 0, 0}; out[qi] = res; return; }
+    if (fabs(fb) < tol) { res = IvResultRec{b, fabs(fb), 0, 0}; out[qi] = res; return; }
+    if (fabs(fa) < fabs(fb)) { double t = a; a = b; b = t; t = fa; fa = fb; fb = t; }
+    double c = a, fc = fa, d = 0.0;
+    bool mflag = true;
+    for (int64_t it = 0; it < cfg.max_iter; ++it) {
+        if (fabs(fb) < tol || fabs(b - a) < tol) {
+            out[qi] = IvResultRec{b, fabs(fb), static_cast<int32_t>(it + 1), 0};
+            return;
+        }
+        double s;
+        if (fa != fc && fb != fc)
+            s = a * fb * fc / ((fa - fb) * (fa - fc)) + b * fa * fc / ((fb - fa) * (fb - fc)) +
+                c * fa * fb / ((fc - fa) * (fc - fb));
+        else
+            s = b - fb * (b - a) / (fb - fa);
+        const double bisect = (3.0 * a + b) / 4.0;
+        const bool c1 = !((s > bisect && s < b) || (s < bisect && s > b));
+        const bool c2 = mflag && fabs(s - b) >= fabs(b - c) / 2.0;
+        const bool c3 = !mflag && fabs(s - b) >= fabs(c - d) / 2.0;
+        const bool c4 = mflag && fabs(b - c) < tol;
+        const bool c5 = !mflag && fabs(c - d) < tol;
+        if (c1 || c2 || c3 || c4 || c5) { s = (a + b) / 2.0; mflag = true; }
+        else mflag = false;
+        const double fs = f(s);
+        if (!isfinite(fs)) {
+            out[qi] = IvResultRec{NAN, NAN, static_cast<int32_t>(it + 1), kIvSolver};
+            return;
+        }
+        d = c; c = b; fc = fb;
+        if (fa * fs < 0.0) { b = s; fb = fs; } else { a = s; fa = fs; }
+        if (fabs(fa) < fabs(fb)) { double t = a; a = b; b = t; t = fa; fa = fb; fb = t; }
+    }
+    out[qi] = IvResultRec{NAN, fabs(fb), static_cast<int32_t>(cfg.max_iter), kIvNoConvergence};
+}
+
+}  // namespace
+
+#define MANGO_TABLE_VARIANTS(X) X(32, 4) X(64, 4) X(64, 7)
+
+bool table_variant_supported(const SolveVariant& v) {
+#define MANGO_IS(K, M) if (v.lanes == K && v.nodes == M) return true;
+    MANGO_TABLE_VARIANTS(MANGO_IS)
+#undef MANGO_IS
+    return false;
+}
+
+int launch_table_snapshots(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, const int32_t* d_steps,
+                           const int32_t* d_n_snap, int max_snap, int row, double* d_x, double* d_snaps,
+                           int32_t* d_status, hipStream_t stream) {
+    if (n_jobs <= 0) return 0;
+#define MANGO_LAUNCH(K, M)                                                                                      \
+    if (v.lanes == K && v.nodes == M) {                                                                         \
+        constexpr int GPB = kSolveBlock / K;                                                                    \
+        hipLaunchKernelGGL((table_snapshot_kernel<K, M>), dim3((n_jobs + GPB - 1) / GPB), dim3(kSolveBlock), 0, \
+                           stream, d_jobs, n_jobs, d_steps, d_n_snap, max_snap, row, d_x, d_snaps, d_status);   \
+        return hipGetLastError() == hipSuccess ? 0 : 1;                                                         \
+    }
+    MANGO_TABLE_VARIANTS(MANGO_LAUNCH)
+#undef MANGO_LAUNCH
+    return 2;
+}
+
+int launch_table_extract(const ExtractArgs& a, hipStream_t stream) {
+    const int64_t n = static_cast<int64_t>(a.n_sigma) * a.n_rate * a.n_tau;
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(table_extract_kernel, dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       stream, a);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_fit_axis(const double* d_src, double* d_dst, int64_t total, int na, int64_t stride, const double* d_lu,
+                    const int32_t* d_piv, const double* d_b, unsigned long long* d_max_res, hipStream_t stream) {
+    if (na <= 0 || total <= 0) return 0;
+    const int64_t slices = total / na;
+    hipLaunchKernelGGL(fit_axis_kernel, dim3(static_cast<unsigned>((slices + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       stream, d_src, d_dst, total, na, stride, d_lu, d_piv, d_b, d_max_res);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_table_price(const DeviceTable& t, const double* d_rows, int64_t n, double* d_price, double* d_vega,
+                       hipStream_t stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(table_price_kernel, dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       stream, t, d_rows, n, d_price, d_vega);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_interp_iv(const DeviceTable& t, const IvConfig& cfg, const IvQueryRec* d_q, int64_t n, IvResultRec* d_out,
+                     hipStream_t stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(interp_iv_kernel, dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       stream, t, cfg, d_q, n, d_out);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+}  // namespace mango_amd

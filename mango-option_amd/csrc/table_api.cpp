@@ -1,0 +1,925 @@
+// SPDX-License-Identifier: MIT
+//
+// Host side of the price-table and interpolated-IV C ABI (include/mango_amd.h):
+// the builder's decisions made exactly as the reference's host code makes
+// them, then the device pipeline of price_table.hip.
+//
+// Reference behaviour mirrored here:
+//   PriceTableBuilderND<4>::build checks       src/option/table/bspline/bspline_builder.cpp:26-134
+//   make_batch, ensure_moneyness_coverage,
+//   estimate_pde_grid (builder)                 bspline_builder.cpp:198-278
+//   solve_batch: explicit-grid constraint check
+//   and the estimated-grid fallback             bspline_builder.cpp:280-371
+//   BatchAmericanOptionSolver routing           src/option/american_option_batch.cpp:26-107, 219-382
+//   convert_times_to_indices                    src/pde/core/grid.hpp:754-836
+//   repair_failed_slices (neighbour search)     bspline_builder.cpp:757-960
+//   from_vectors (sort + dedupe)                bspline_builder.cpp:534-594
+//   make_price_table (continuous, B-spline)     src/option/price_table_factory.cpp:262-327
+//   InterpolatedIVSolver::create bounds check   src/option/interpolated_iv_solver.hpp:328-362
+//   C ABI conversions and status mapping        src/ffi/mango_c_api.cpp:44-260, 467-730
+#include "../../include/mango_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "american_kernels.hpp"
+#include "bspline_basis.hpp"
+#include "mango_grid.hpp"
+#include "mango_host.hpp"
+#include "price_table.hpp"
+
+namespace mango_amd {
+namespace {
+
+constexpr double kFitTolerance = 1e-6;
+constexpr int kTableMaxNodes = 64 * 7;
+
+// One device allocation carved into 256-byte aligned pieces.
+class DeviceArena {
+public:
+    ~DeviceArena() { if (base_) (void)hipFree(base_); }
+    template <typename T>
+    size_t reserve(size_t count) {
+        const size_t off = size_;
+        size_ += (count * sizeof(T) + 255) / 256 * 256;
+        return off;
+    }
+    bool allocate() { return size_ == 0 || hipMalloc(&base_, size_) == hipSuccess; }
+    template <typename T>
+    T* at(size_t off) const { return reinterpret_cast<T*>(static_cast<char*>(base_) + off); }
+    void* release() { void* p = base_; base_ = nullptr; return p; }
+
+private:
+    void* base_ = nullptr;
+    size_t size_ = 0;
+};
+
+struct TableData {
+    std::array<std::vector<double>, kTableDims> axes;
+    double K_ref = 0.0, q = 0.0;
+    int32_t type = MANGO_PUT;
+    size_t size = 0;
+    void* mem = nullptr;  // axes, knots, coefficients, pre-EEP prices
+    DeviceTable view{};
+    double* d_prices = nullptr;
+    MangoTableBuildInfo info{};
+    ~TableData() { if (mem) (void)hipFree(mem); }
+};
+
+std::vector<double> sort_dedupe(std::vector<double> v) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    return v;
+}
+
+// 1-D collocation system of one axis: B[i][j] = N_j(x_i), dense LU with
+// partial pivoting (row swaps in piv), the factors every slice reuses.
+struct Collocation {
+    std::vector<double> B, LU;
+    std::vector<int32_t> piv;
+};
+
+bool collocation(const std::vector<double>& x, Collocation& c) {
+    const int n = static_cast<int>(x.size());
+    if (n < 4) return false;
+    for (int i = 1; i < n; ++i)
+        if (x[static_cast<size_t>(i)] - x[static_cast<size_t>(i - 1)] < 1e-14) return false;
+    const std::vector<double> t = clamped_knots(x);
+    c.B.assign(static_cast<size_t>(n) * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        const double xi = x[static_cast<size_t>(i)];
+        const int span = find_span(t.data(), n, xi);
+        double w[4];
+        cubic_basis(t.data(), n + 4, span, xi, w, false);
+        for (int k = 0; k < 4; ++k) {
+            const int col = span - k;
+            if (col >= 0 && col < n) c.B[static_cast<size_t>(i) * n + col] = w[k];
+        }
+    }
+    c.LU = c.B;
+    c.piv.assign(static_cast<size_t>(n), 0);
+    auto L = [&](int r, int col) -> double& { return c.LU[static_cast<size_t>(r) * n + col]; };
+    for (int col = 0; col < n; ++col) {
+        int p = col;
+        for (int r = col + 1; r < n; ++r) if (std::abs(L(r, col)) > std::abs(L(p, col))) p = r;
+        c.piv[static_cast<size_t>(col)] = p;
+        if (L(p, col) == 0.0) return false;
+        if (p != col) for (int j = 0; j < n; ++j) std::swap(L(col, j), L(p, j));
+        for (int r = col + 1; r < n; ++r) {
+            const double f = L(r, col) / L(col, col);
+            L(r, col) = f;
+            for (int j = col + 1; j < n; ++j) L(r, j) -= f * L(col, j);
+        }
+    }
+    return true;
+}
+
+// convert_times_to_indices, uniform time grid from 0 to T in n steps:
+// nearest step (round half up), clamped to n, sorted and deduplicated.
+bool snapshot_steps(const std::vector<double>& times, double T, int64_t n, std::vector<int32_t>& out) {
+    out.clear();
+    if (n <= 0) return false;
+    const double dt = (T - 0.0) / static_cast<double>(n);
+    if (!(dt > 0.0)) return false;
+    std::vector<int64_t> idx;
+    for (double t : times) {
+        if (t < 0.0 || t > T) return false;
+        const double step_exact = (t - 0.0) / dt;
+        int64_t si = static_cast<int64_t>(std::floor(step_exact + 0.5));
+        idx.push_back(std::min(si, n));
+    }
+    std::sort(idx.begin(), idx.end());
+    idx.erase(std::unique(idx.begin(), idx.end()), idx.end());
+    for (int64_t i : idx) out.push_back(static_cast<int32_t>(i));
+    return true;
+}
+
+SolveVariant table_variant(int64_t n) {
+    if (n <= 128) return {32, 4};
+    if (n <= 256) return {64, 4};
+    return {64, 7};
+}
+
+MangoPricingParams batch_param(double S, double K, double T, double r, double q, double sigma, int32_t type) {
+    MangoPricingParams p{};
+    p.spot = S;
+    p.strike = K;
+    p.maturity = T;
+    p.rate_const = r;
+    p.dividend_yield = q;
+    p.volatility = sigma;
+    p.option_type = type;
+    return p;
+}
+
+MangoStatus fail(MangoError* err, MangoStatus code, const std::string& msg) {
+    set_err(err, code, msg.c_str());
+    return code;
+}
+
+// PriceTableBuilderND<4>::build with the EEP transform.
+MangoStatus build_table(std::array<std::vector<double>, kTableDims> axes, double K_ref, double q, int32_t type,
+                        const MangoGridConfig* pde_grid, double max_failure_rate, std::shared_ptr<TableData>& out,
+                        MangoError* err) {
+    const auto t_start = std::chrono::steady_clock::now();
+    if (!device_ready(err)) return MANGO_ERR_SOLVER;
+    if (type != MANGO_CALL && type != MANGO_PUT) return fail(err, MANGO_ERR_VALIDATION, "invalid option_type");
+    if (!(max_failure_rate >= 0.0 && max_failure_rate <= 1.0))
+        return fail(err, MANGO_ERR_VALIDATION, "max_failure_rate must be in [0, 1]");
+    for (auto& a : axes) {
+        for (double v : a)
+            if (!std::isfinite(v)) return fail(err, MANGO_ERR_VALIDATION, "table axes must be finite");
+        a = sort_dedupe(std::move(a));
+    }
+    for (int d = 0; d < kTableDims; ++d)
+        if (axes[static_cast<size_t>(d)].size() < 4)
+            return fail(err, MANGO_ERR_VALIDATION, "each table axis needs at least 4 points");
+    const auto& lm = axes[0];
+    const auto& tau = axes[1];
+    const auto& sig = axes[2];
+    const auto& rate = axes[3];
+    if (!(tau.front() > 0.0)) return fail(err, MANGO_ERR_VALIDATION, "maturities must be positive");
+    if (!(sig.front() > 0.0)) return fail(err, MANGO_ERR_VALIDATION, "volatilities must be positive");
+    if (!(K_ref > 0.0) || !std::isfinite(K_ref)) return fail(err, MANGO_ERR_VALIDATION, "K_ref must be positive");
+    if (!(q >= 0.0) || !std::isfinite(q)) return fail(err, MANGO_ERR_VALIDATION, "invalid dividend yield");
+    const int32_t mode = pde_grid ? pde_grid->mode : MANGO_GRID_MODE_ACCURACY;
+    if (mode != MANGO_GRID_MODE_ACCURACY && mode != MANGO_GRID_MODE_EXPLICIT)
+        return fail(err, MANGO_ERR_VALIDATION, "the table build takes an EXPLICIT or ACCURACY grid config");
+    GridDesc espec{};
+    int64_t en_time = 0;
+    if (mode == MANGO_GRID_MODE_EXPLICIT) {
+        espec = to_desc(pde_grid->spec);
+        en_time = static_cast<int64_t>(pde_grid->n_time);
+        if (espec.n_points < 3 || en_time <= 0)
+            return fail(err, MANGO_ERR_VALIDATION, "explicit grid needs n_points >= 3 and n_time > 0");
+        if (lm.front() < espec.x_min || lm.back() > espec.x_max)
+            return fail(err, MANGO_ERR_VALIDATION, "explicit PDE grid does not cover the log-moneyness axis");
+    }
+
+    // make_batch: sigma outer, rate inner, S = K = K_ref, T = largest maturity
+    const size_t Nm = lm.size(), Nt = tau.size(), Ns = sig.size(), Nr = rate.size();
+    const size_t n_slices = Ns * Nr;
+    const double T = tau.back();
+    std::vector<MangoPricingParams> batch;
+    batch.reserve(n_slices);
+    for (double s : sig)
+        for (double r : rate) batch.push_back(batch_param(K_ref, K_ref, T, r, q, s, type));
+    double max_sst = 0.0;
+    for (const auto& p : batch) max_sst = std::max(max_sst, p.volatility * std::sqrt(p.maturity));
+    auto ensure_coverage = [&](Accuracy& a) {
+        const double half = std::max(std::abs(lm.front()), std::abs(lm.back()));
+        a.n_sigma = std::max(a.n_sigma, (half / std::max(max_sst, 1e-10)) * 1.1);
+    };
+
+    // solve_batch: the grid each solve uses
+    Accuracy solver_acc = to_accuracy(nullptr);
+    bool have_custom = false, fallback = false;
+    GridDesc custom{};
+    int64_t custom_nt = 0;
+    if (mode == MANGO_GRID_MODE_EXPLICIT) {
+        double max_dx;
+        const double width = espec.x_max - espec.x_min;
+        if (espec.type == kUniform) {
+            max_dx = width / static_cast<double>(espec.n_points - 1);
+        } else {
+            std::vector<double> xg(static_cast<size_t>(espec.n_points));
+            generate_grid(espec, xg.data());
+            max_dx = 0.0;
+            for (size_t i = 1; i < xg.size(); ++i) max_dx = std::max(max_dx, xg[i] - xg[i - 1]);
+        }
+        if (width <= 5.8 && max_dx <= 0.05 && width >= 6.0 * max_sst) {
+            have_custom = true;
+            custom = espec;
+            custom_nt = en_time;
+        } else {
+            fallback = true;
+            Accuracy a = to_accuracy(nullptr);
+            const int64_t clamped = std::clamp<int64_t>(espec.n_points, 100, 1200);
+            a.min_spatial_points = clamped;
+            a.max_spatial_points = clamped;
+            a.max_time_steps = en_time;
+            if (espec.type == kSinh) a.alpha = espec.concentration;
+            const double max_abs_x = std::max(std::abs(espec.x_min), std::abs(espec.x_max));
+            a.n_sigma = std::max(5.0, (max_abs_x / std::max(max_sst, 1e-10)) * 1.1);
+            ensure_coverage(a);
+            solver_acc = a;
+        }
+    } else {
+        Accuracy a = to_accuracy(pde_grid ? &pde_grid->accuracy : nullptr);
+        if (a.tol == 0.0) a = to_accuracy(nullptr);
+        ensure_coverage(a);
+        auto est = estimate_batch_grid_host(batch.data(), batch.size(), a, nullptr);
+        have_custom = true;
+        custom = est.first;
+        custom_nt = est.second;
+    }
+    if (solver_acc.tol == 0.0) solver_acc = to_accuracy(nullptr);
+
+    // is_normalized_eligible (one option type, S and K positive, the first
+    // option's estimated grid narrow and fine enough, margins around ln(S/K))
+    bool normalized = true;
+    {
+        const GridDesc g0 = estimate_spatial(batch[0].spot, batch[0].strike, batch[0].maturity, batch[0].volatility,
+                                             solver_acc);
+        const double dx = (g0.x_max - g0.x_min) / static_cast<double>(g0.n_points - 1);
+        if (dx > 0.05 || g0.x_max - g0.x_min > 5.8) normalized = false;
+        const double m = 0.0;  // ln(K_ref / K_ref)
+        const double mm = std::max(0.35, 6.0 * dx);
+        if (m - g0.x_min < mm || g0.x_max - m < mm) normalized = false;
+    }
+
+    // jobs: one per PDE group (normalized: (sigma, r) within 1e-10) or per slice
+    std::vector<PdeJob> jobs;
+    std::vector<int32_t> job_of(n_slices, -1);
+    auto make = [&](const MangoPricingParams& p, const GridDesc& g, int64_t nt) {
+        PdeJob j{};
+        j.maturity = p.maturity;
+        j.volatility = p.volatility;
+        j.rate = p.rate_const;
+        j.dividend_yield = p.dividend_yield;
+        j.spot = p.spot;
+        j.strike = p.strike;
+        j.c_t = solver_acc.c_t;
+        j.grid = g;
+        j.option_type = p.option_type == MANGO_PUT ? 1 : 0;
+        j.n_time = static_cast<int32_t>(nt);
+        j.out_index = static_cast<int32_t>(jobs.size());
+        return j;
+    };
+    if (normalized) {
+        constexpr double TOL = 1e-10;
+        std::vector<MangoPricingParams> reps;
+        for (size_t s = 0; s < n_slices; ++s) {
+            const auto& p = batch[s];
+            int found = -1;
+            for (size_t g = 0; g < reps.size() && found < 0; ++g)
+                if (std::abs(reps[g].volatility - p.volatility) < TOL && std::abs(reps[g].rate_const - p.rate_const) < TOL)
+                    found = static_cast<int>(g);
+            if (found < 0) {
+                const MangoPricingParams np = batch_param(1.0, 1.0, p.maturity, p.rate_const, p.dividend_yield,
+                                                          p.volatility, p.option_type);
+                GridDesc g;
+                int64_t nt;
+                if (have_custom) { g = custom; nt = custom_nt; }
+                else { auto e = estimate_batch_grid_host(&np, 1, solver_acc, nullptr); g = e.first; nt = e.second; }
+                found = static_cast<int>(reps.size());
+                reps.push_back(np);
+                jobs.push_back(make(np, g, nt));
+            }
+            job_of[s] = found;
+        }
+    } else {
+        GridDesc g;
+        int64_t nt;
+        if (have_custom) { g = custom; nt = custom_nt; }
+        else { auto e = estimate_batch_grid_host(batch.data(), batch.size(), solver_acc, nullptr); g = e.first; nt = e.second; }
+        for (size_t s = 0; s < n_slices; ++s) {
+            jobs.push_back(make(batch[s], g, nt));
+            job_of[s] = static_cast<int32_t>(s);
+        }
+    }
+    const int n_jobs = static_cast<int>(jobs.size());
+    int64_t row = 3;
+    for (const auto& j : jobs) row = std::max(row, j.grid.n_points);
+    if (row > kTableMaxNodes)
+        return fail(err, MANGO_ERR_VALIDATION, "PDE grid larger than the device table build supports (448 points)");
+    const SolveVariant var = table_variant(row);
+
+    // snapshot plan per job; a job whose times fall outside its time domain fails
+    const int max_snap = static_cast<int>(Nt);
+    std::vector<int32_t> steps(static_cast<size_t>(n_jobs) * max_snap, -1), n_snap(static_cast<size_t>(n_jobs), 0);
+    std::vector<char> plan_ok(static_cast<size_t>(n_jobs), 1);
+    for (int j = 0; j < n_jobs; ++j) {
+        std::vector<int32_t> st;
+        const char* why = nullptr;
+        MangoPricingParams p = batch_param(jobs[j].spot, jobs[j].strike, jobs[j].maturity, jobs[j].rate,
+                                           jobs[j].dividend_yield, jobs[j].volatility, type);
+        if (validate_params(p, &why) != MANGO_OK || !snapshot_steps(tau, jobs[j].maturity, jobs[j].n_time, st)) {
+            plan_ok[static_cast<size_t>(j)] = 0;
+            continue;
+        }
+        n_snap[static_cast<size_t>(j)] = static_cast<int32_t>(st.size());
+        std::copy(st.begin(), st.end(), steps.begin() + static_cast<std::ptrdiff_t>(j) * max_snap);
+    }
+
+    // device workspace
+    const size_t size = Nm * Nt * Ns * Nr;
+    DeviceArena ws;
+    const size_t o_jobs = ws.reserve<PdeJob>(jobs.size());
+    const size_t o_steps = ws.reserve<int32_t>(steps.size());
+    const size_t o_nsnap = ws.reserve<int32_t>(n_snap.size());
+    const size_t o_nspace = ws.reserve<int64_t>(jobs.size());
+    const size_t o_status = ws.reserve<int32_t>(jobs.size());
+    const size_t o_x = ws.reserve<double>(static_cast<size_t>(n_jobs) * row);
+    const size_t o_snaps = ws.reserve<double>(static_cast<size_t>(n_jobs) * max_snap * row);
+    const size_t o_src = ws.reserve<int32_t>(n_slices);
+    const size_t o_axes = ws.reserve<double>(Nm + Nt + Ns + Nr);
+    const size_t o_scr = ws.reserve<double>(n_slices * Nt * 2 * static_cast<size_t>(row));
+    const size_t o_a = ws.reserve<double>(size);
+    const size_t o_b = ws.reserve<double>(size);
+    size_t max_axis = 0;
+    for (const auto& a : axes) max_axis = std::max(max_axis, a.size());
+    const size_t o_lu = ws.reserve<double>(max_axis * max_axis * kTableDims);
+    const size_t o_bm = ws.reserve<double>(max_axis * max_axis * kTableDims);
+    const size_t o_piv = ws.reserve<int32_t>(max_axis * kTableDims);
+    const size_t o_res = ws.reserve<unsigned long long>(1);
+    if (!ws.allocate()) return fail(err, MANGO_ERR_SOLVER, "hipMalloc failed (table build workspace)");
+
+    // the table itself: axes, knots, coefficients, pre-EEP prices
+    auto td = std::make_shared<TableData>();
+    td->axes = axes;
+    td->K_ref = K_ref;
+    td->q = q;
+    td->type = type;
+    td->size = size;
+    DeviceArena tm;
+    size_t o_ax[kTableDims], o_kn[kTableDims];
+    for (int d = 0; d < kTableDims; ++d) {
+        o_ax[d] = tm.reserve<double>(axes[static_cast<size_t>(d)].size());
+        o_kn[d] = tm.reserve<double>(axes[static_cast<size_t>(d)].size() + 4);
+    }
+    const size_t o_coef = tm.reserve<double>(size);
+    const size_t o_pr = tm.reserve<double>(size);
+    if (!tm.allocate()) return fail(err, MANGO_ERR_SOLVER, "hipMalloc failed (table)");
+
+    hipStream_t stream = nullptr;
+    auto hcheck = [&](hipError_t e, const char* what) {
+        if (e == hipSuccess) return true;
+        set_err(err, MANGO_ERR_SOLVER, (std::string(what) + ": " + hipGetErrorString(e)).c_str());
+        return false;
+    };
+    std::vector<int64_t> nspace(static_cast<size_t>(n_jobs));
+    for (int j = 0; j < n_jobs; ++j) nspace[static_cast<size_t>(j)] = jobs[j].grid.n_points;
+    // jobs whose snapshot plan failed are not solved: an invalid record fails in the kernel
+    std::vector<PdeJob> launch_jobs = jobs;
+    for (int j = 0; j < n_jobs; ++j)
+        if (!plan_ok[static_cast<size_t>(j)]) launch_jobs[static_cast<size_t>(j)].volatility = -1.0;
+    if (!hcheck(hipMemcpy(ws.at<PdeJob>(o_jobs), launch_jobs.data(), jobs.size() * sizeof(PdeJob), hipMemcpyHostToDevice), "copy jobs") ||
+        !hcheck(hipMemcpy(ws.at<int32_t>(o_steps), steps.data(), steps.size() * sizeof(int32_t), hipMemcpyHostToDevice), "copy steps") ||
+        !hcheck(hipMemcpy(ws.at<int32_t>(o_nsnap), n_snap.data(), n_snap.size() * sizeof(int32_t), hipMemcpyHostToDevice), "copy counts") ||
+        !hcheck(hipMemcpy(ws.at<int64_t>(o_nspace), nspace.data(), nspace.size() * sizeof(int64_t), hipMemcpyHostToDevice), "copy sizes"))
+        return MANGO_ERR_SOLVER;
+    if (launch_table_snapshots(var, ws.at<PdeJob>(o_jobs), n_jobs, ws.at<int32_t>(o_steps), ws.at<int32_t>(o_nsnap),
+                               max_snap, static_cast<int>(row), ws.at<double>(o_x), ws.at<double>(o_snaps),
+                               ws.at<int32_t>(o_status), stream) != 0)
+        return fail(err, MANGO_ERR_SOLVER, "snapshot solve launch failed");
+    std::vector<int32_t> status(static_cast<size_t>(n_jobs));
+    if (!hcheck(hipMemcpy(status.data(), ws.at<int32_t>(o_status), status.size() * sizeof(int32_t), hipMemcpyDeviceToHost), "snapshot solves"))
+        return MANGO_ERR_SOLVER;
+
+    // failure rate over the batch entries, then the repair plan
+    std::vector<char> valid(n_slices, 1);
+    size_t failed = 0;
+    for (size_t s = 0; s < n_slices; ++s) {
+        const int j = job_of[s];
+        if (status[static_cast<size_t>(j)] != kStatusOk || !plan_ok[static_cast<size_t>(j)] || n_snap[static_cast<size_t>(j)] < 1) {
+            valid[s] = 0;
+            ++failed;
+        }
+    }
+    if (static_cast<double>(failed) / static_cast<double>(n_slices) > max_failure_rate) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "price table build: %zu of %zu PDE solves failed", failed, n_slices);
+        return fail(err, MANGO_ERR_VALIDATION, msg);
+    }
+    std::vector<int32_t> src(n_slices);
+    uint64_t partial = 0, repaired_full = 0;
+    for (size_t s = 0; s < n_slices; ++s) {
+        src[s] = job_of[s];
+        if (valid[s]) partial += static_cast<uint64_t>(Nt - static_cast<size_t>(n_snap[static_cast<size_t>(job_of[s])]));
+    }
+    for (size_t k = 0; k < n_slices; ++k) {
+        if (valid[k]) continue;
+        // nearest valid (sigma, r) slice by L1 distance, sigma offset from
+        // -d to d, rate offset - then +; repaired slices become donors
+        const long s0 = static_cast<long>(k / Nr), r0 = static_cast<long>(k % Nr);
+        long ns = -1, nr = -1;
+        const long max_dist = static_cast<long>(Ns - 1 + Nr - 1);
+        for (long d = 1; d <= max_dist && ns < 0; ++d)
+            for (long ds = -d; ds <= d && ns < 0; ++ds) {
+                const long dr = d - std::abs(ds);
+                for (long sign : {-1L, 1L}) {
+                    const long a = s0 + ds, b = r0 + sign * dr;
+                    if (a >= 0 && a < static_cast<long>(Ns) && b >= 0 && b < static_cast<long>(Nr) &&
+                        valid[static_cast<size_t>(a) * Nr + static_cast<size_t>(b)]) {
+                        ns = a; nr = b;
+                        break;
+                    }
+                }
+            }
+        if (ns < 0) return fail(err, MANGO_ERR_VALIDATION, "price table build: no valid slice to repair from");
+        src[k] = src[static_cast<size_t>(ns) * Nr + static_cast<size_t>(nr)];
+        valid[k] = 1;
+        ++repaired_full;
+    }
+
+    // extraction + EEP into buffer a
+    std::vector<double> ax_cat;
+    for (const auto& a : axes) ax_cat.insert(ax_cat.end(), a.begin(), a.end());
+    if (!hcheck(hipMemcpy(ws.at<int32_t>(o_src), src.data(), src.size() * sizeof(int32_t), hipMemcpyHostToDevice), "copy repair map") ||
+        !hcheck(hipMemcpy(ws.at<double>(o_axes), ax_cat.data(), ax_cat.size() * sizeof(double), hipMemcpyHostToDevice), "copy axes"))
+        return MANGO_ERR_SOLVER;
+    ExtractArgs ea{};
+    ea.lm = ws.at<double>(o_axes);
+    ea.tau = ea.lm + Nm;
+    ea.sigma = ea.tau + Nt;
+    ea.rate = ea.sigma + Ns;
+    ea.n_m = static_cast<int32_t>(Nm);
+    ea.n_tau = static_cast<int32_t>(Nt);
+    ea.n_sigma = static_cast<int32_t>(Ns);
+    ea.n_rate = static_cast<int32_t>(Nr);
+    ea.src_job = ws.at<int32_t>(o_src);
+    ea.n_snap = ws.at<int32_t>(o_nsnap);
+    ea.n_space = ws.at<int64_t>(o_nspace);
+    ea.max_snap = max_snap;
+    ea.row = static_cast<int32_t>(row);
+    ea.x = ws.at<double>(o_x);
+    ea.snaps = ws.at<double>(o_snaps);
+    ea.spline_scratch = ws.at<double>(o_scr);
+    ea.K_ref = K_ref;
+    ea.dividend_yield = q;
+    ea.option_type = type == MANGO_PUT ? 1 : 0;
+    ea.raw = tm.at<double>(o_pr);
+    ea.eep = ws.at<double>(o_a);
+    if (launch_table_extract(ea, stream) != 0) return fail(err, MANGO_ERR_SOLVER, "extraction launch failed");
+
+    // separable fit, axes 3 .. 0, ping-pong between buffers a and b
+    std::vector<double> lu_all, b_all;
+    std::vector<int32_t> piv_all;
+    size_t o_lu_ax[kTableDims], o_piv_ax[kTableDims];
+    for (int d = 0; d < kTableDims; ++d) {
+        Collocation c;
+        if (!collocation(axes[static_cast<size_t>(d)], c))
+            return fail(err, MANGO_ERR_VALIDATION, "B-spline collocation matrix is singular for a table axis");
+        o_lu_ax[d] = lu_all.size();
+        o_piv_ax[d] = piv_all.size();
+        lu_all.insert(lu_all.end(), c.LU.begin(), c.LU.end());
+        b_all.insert(b_all.end(), c.B.begin(), c.B.end());
+        piv_all.insert(piv_all.end(), c.piv.begin(), c.piv.end());
+    }
+    const unsigned long long zero = 0;
+    if (!hcheck(hipMemcpy(ws.at<double>(o_lu), lu_all.data(), lu_all.size() * sizeof(double), hipMemcpyHostToDevice), "copy LU") ||
+        !hcheck(hipMemcpy(ws.at<double>(o_bm), b_all.data(), b_all.size() * sizeof(double), hipMemcpyHostToDevice), "copy B") ||
+        !hcheck(hipMemcpy(ws.at<int32_t>(o_piv), piv_all.data(), piv_all.size() * sizeof(int32_t), hipMemcpyHostToDevice), "copy pivots") ||
+        !hcheck(hipMemcpy(ws.at<unsigned long long>(o_res), &zero, sizeof zero, hipMemcpyHostToDevice), "clear residual"))
+        return MANGO_ERR_SOLVER;
+    double* cur = ws.at<double>(o_a);
+    double* nxt = ws.at<double>(o_b);
+    int64_t stride = 1;
+    for (int d = kTableDims - 1; d >= 0; --d) {
+        const int na = static_cast<int>(axes[static_cast<size_t>(d)].size());
+        if (launch_fit_axis(cur, nxt, static_cast<int64_t>(size), na, stride, ws.at<double>(o_lu) + o_lu_ax[d],
+                            ws.at<int32_t>(o_piv) + o_piv_ax[d], ws.at<double>(o_bm) + o_lu_ax[d],
+                            ws.at<unsigned long long>(o_res), stream) != 0)
+            return fail(err, MANGO_ERR_SOLVER, "fit launch failed");
+        std::swap(cur, nxt);
+        stride *= na;
+    }
+    unsigned long long res_bits = 0;
+    if (!hcheck(hipMemcpy(tm.at<double>(o_coef), cur, size * sizeof(double), hipMemcpyDeviceToDevice), "copy coefficients") ||
+        !hcheck(hipMemcpy(&res_bits, ws.at<unsigned long long>(o_res), sizeof res_bits, hipMemcpyDeviceToHost), "fit residual"))
+        return MANGO_ERR_SOLVER;
+    double max_res;
+    std::memcpy(&max_res, &res_bits, sizeof max_res);
+    if (!(max_res <= kFitTolerance)) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "price table fit residual %.3e exceeds %.1e", max_res, kFitTolerance);
+        return fail(err, MANGO_ERR_VALIDATION, msg);
+    }
+
+    // axes and knots into the table memory
+    for (int d = 0; d < kTableDims; ++d) {
+        const auto& a = axes[static_cast<size_t>(d)];
+        const std::vector<double> kn = clamped_knots(a);
+        if (!hcheck(hipMemcpy(tm.at<double>(o_ax[d]), a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice), "copy axis") ||
+            !hcheck(hipMemcpy(tm.at<double>(o_kn[d]), kn.data(), kn.size() * sizeof(double), hipMemcpyHostToDevice), "copy knots"))
+            return MANGO_ERR_SOLVER;
+        td->view.grid[d] = tm.at<double>(o_ax[d]);
+        td->view.knots[d] = tm.at<double>(o_kn[d]);
+        td->view.n[d] = static_cast<int32_t>(a.size());
+    }
+    td->view.coeffs = tm.at<double>(o_coef);
+    td->view.K_ref = K_ref;
+    td->view.dividend_yield = q;
+    td->view.option_type = type == MANGO_PUT ? 1 : 0;
+    td->d_prices = tm.at<double>(o_pr);
+    td->mem = tm.release();
+    auto& info = td->info;
+    info.n_pde_solves = n_slices - failed;
+    info.failed_pde = failed;
+    info.repaired_full_slices = repaired_full;
+    info.repaired_partial_points = partial;
+    info.fallback_grid = fallback ? 1 : 0;
+    info.max_fit_residual = max_res;
+    info.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    out = std::move(td);
+    return MANGO_OK;
+}
+
+// ---- queries ----
+
+IvConfig to_iv_config(const MangoInterpSolverConfig* c) {
+    IvConfig cfg{50, 1e-6, 0.01, 3.0, 1e-4};  // InterpolatedIVSolverConfig defaults
+    if (c) {
+        cfg.max_iter = static_cast<int64_t>(c->max_iter);
+        cfg.tolerance = c->tolerance;
+        cfg.sigma_min = c->sigma_min;
+        cfg.sigma_max = c->sigma_max;
+        cfg.vega_threshold = c->vega_threshold;
+    }
+    return cfg;
+}
+
+// build_iv_query (mango_c_api.cpp:168-204) for the device record; false
+// with *why set when the C struct cannot become a query.
+bool to_query(const MangoIvQuery& q, IvQueryRec& rec, const char** why) {
+    if (q.option_type != MANGO_CALL && q.option_type != MANGO_PUT) { *why = "invalid option_type"; return false; }
+    if (q.n_tenor_points != 0) { *why = "yield curves are not supported by the GPU interpolated IV solver"; return false; }
+    if (q.dividends == nullptr && q.n_dividends > 0) { *why = "dividends is null but n_dividends > 0"; return false; }
+    if (!std::isfinite(q.dividend_yield)) { *why = "dividend_yield must be finite"; return false; }
+    if (!std::isfinite(q.rate_const)) { *why = "rate must be finite"; return false; }
+    for (uint64_t i = 0; i < q.n_dividends; ++i) {
+        const double ct = q.dividends[i].calendar_time, amt = q.dividends[i].amount;
+        if (!std::isfinite(ct) || ct < 0.0 || ct > q.maturity || !std::isfinite(amt) || amt < 0.0) {
+            *why = "invalid discrete dividend";
+            return false;
+        }
+    }
+    rec = IvQueryRec{q.spot, q.strike, q.maturity, q.rate_const, q.dividend_yield, q.market_price, q.option_type, 0};
+    return true;
+}
+
+}  // namespace
+}  // namespace mango_amd
+
+using namespace mango_amd;
+
+struct MangoPriceTable {
+    std::shared_ptr<TableData> t;
+};
+struct MangoInterpIvSolver {
+    std::shared_ptr<TableData> t;
+    IvConfig cfg;
+};
+
+namespace {
+
+std::mutex g_query_mu;
+
+MangoStatus make_solver(const std::shared_ptr<TableData>& t, const MangoInterpSolverConfig* cfg,
+                        MangoInterpIvSolver** out, MangoError* err) {
+    for (const auto& a : t->axes)
+        if (!(a.front() < a.back())) return fail(err, MANGO_ERR_VALIDATION, "table axis range is empty");
+    *out = new (std::nothrow) MangoInterpIvSolver{t, to_iv_config(cfg)};
+    if (!*out) return fail(err, MANGO_ERR_SOLVER, "out of memory");
+    return MANGO_OK;
+}
+
+// IVSolverFactoryConfig -> axes; adaptive and segmented tables are not built here.
+MangoStatus factory_table(const MangoIvFactoryConfig* c, std::shared_ptr<TableData>& t, MangoError* err) {
+    if (c->option_type != MANGO_CALL && c->option_type != MANGO_PUT) return fail(err, MANGO_ERR_VALIDATION, "invalid option_type");
+    if (!std::isfinite(c->spot)) return fail(err, MANGO_ERR_VALIDATION, "spot must be finite");
+    if (!std::isfinite(c->dividend_yield)) return fail(err, MANGO_ERR_VALIDATION, "dividend_yield must be finite");
+    if (c->adaptive) return fail(err, MANGO_ERR_VALIDATION, "adaptive grid refinement is not supported by the GPU table build");
+    if (c->discrete_dividends) return fail(err, MANGO_ERR_VALIDATION, "discrete-dividend tables are not supported by the GPU table build");
+    auto vec = [](const double* p, uint64_t n) { return (p && n) ? std::vector<double>(p, p + n) : std::vector<double>{}; };
+    std::array<std::vector<double>, kTableDims> axes;
+    for (double m : vec(c->moneyness, c->n_moneyness)) {
+        if (!(m > 0.0) || !std::isfinite(m)) return fail(err, MANGO_ERR_VALIDATION, "moneyness must be positive and finite");
+        axes[0].push_back(std::log(m));
+    }
+    axes[1] = vec(c->maturity_grid, c->n_maturity);
+    axes[2] = vec(c->vol, c->n_vol);
+    axes[3] = vec(c->rate, c->n_rate);
+    return build_table(std::move(axes), c->spot, c->dividend_yield, c->option_type, nullptr, 0.0, t, err);
+}
+
+// Runs the IV kernel over host queries (synchronous).
+MangoStatus run_iv(const MangoInterpIvSolver* s, const std::vector<IvQueryRec>& q, std::vector<IvResultRec>& r,
+                   MangoError* err) {
+    r.assign(q.size(), IvResultRec{});
+    if (q.empty()) return MANGO_OK;
+    if (!device_ready(err)) return MANGO_ERR_SOLVER;
+    DeviceArena a;
+    const size_t oq = a.reserve<IvQueryRec>(q.size()), orr = a.reserve<IvResultRec>(q.size());
+    if (!a.allocate()) return fail(err, MANGO_ERR_SOLVER, "hipMalloc failed (IV queries)");
+    if (hipMemcpy(a.at<IvQueryRec>(oq), q.data(), q.size() * sizeof(IvQueryRec), hipMemcpyHostToDevice) != hipSuccess ||
+        launch_interp_iv(s->t->view, s->cfg, a.at<IvQueryRec>(oq), static_cast<int64_t>(q.size()), a.at<IvResultRec>(orr),
+                         nullptr) != 0 ||
+        hipMemcpy(r.data(), a.at<IvResultRec>(orr), r.size() * sizeof(IvResultRec), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(err, MANGO_ERR_SOLVER, "interpolated IV kernel failed");
+    return MANGO_OK;
+}
+
+MangoStatus run_price(const TableData& t, const std::vector<double>& rows, std::vector<double>& price,
+                      std::vector<double>* vega, MangoError* err) {
+    const size_t n = rows.size() / 5;
+    price.assign(n, std::numeric_limits<double>::quiet_NaN());
+    if (vega) vega->assign(n, std::numeric_limits<double>::quiet_NaN());
+    if (n == 0) return MANGO_OK;
+    if (!device_ready(err)) return MANGO_ERR_SOLVER;
+    DeviceArena a;
+    const size_t orow = a.reserve<double>(rows.size()), op = a.reserve<double>(n), ov = a.reserve<double>(n);
+    if (!a.allocate()) return fail(err, MANGO_ERR_SOLVER, "hipMalloc failed (table points)");
+    if (hipMemcpy(a.at<double>(orow), rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        launch_table_price(t.view, a.at<double>(orow), static_cast<int64_t>(n), a.at<double>(op),
+                           vega ? a.at<double>(ov) : nullptr, nullptr) != 0 ||
+        hipMemcpy(price.data(), a.at<double>(op), n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        (vega && hipMemcpy(vega->data(), a.at<double>(ov), n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+        return fail(err, MANGO_ERR_SOLVER, "table price kernel failed");
+    return MANGO_OK;
+}
+
+bool price_row(const MangoPricingParams* p, double* row) {
+    if (!p || p->n_tenor_points != 0 || (p->dividends == nullptr && p->n_dividends > 0)) return false;
+    if (p->option_type != MANGO_CALL && p->option_type != MANGO_PUT) return false;
+    row[0] = p->spot; row[1] = p->strike; row[2] = p->maturity; row[3] = p->volatility; row[4] = p->rate_const;
+    return true;
+}
+
+double price_one(const MangoPriceTable* t, const MangoPricingParams* p, bool want_vega) {
+    double row[5];
+    if (!t || !price_row(p, row)) return std::numeric_limits<double>::quiet_NaN();
+    std::lock_guard<std::mutex> lk(g_query_mu);
+    std::vector<double> pr, vg;
+    if (run_price(*t->t, std::vector<double>(row, row + 5), pr, want_vega ? &vg : nullptr, nullptr) != MANGO_OK)
+        return std::numeric_limits<double>::quiet_NaN();
+    return want_vega ? vg[0] : pr[0];
+}
+
+const char* iv_status_message(int32_t st) {
+    switch (st) {
+        case MANGO_ERR_VALIDATION: return "interpolated IV: invalid query or outside the table";
+        case MANGO_ERR_ARBITRAGE: return "interpolated IV: price violates arbitrage bounds";
+        case MANGO_ERR_NO_CONVERGENCE: return "interpolated IV: Brent did not converge";
+        case MANGO_ERR_BRACKETING: return "interpolated IV: no sign change over the volatility bracket";
+        default: return "interpolated IV: vega too small or numerical failure";
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+MangoStatus mango_price_table_build(const double* log_moneyness, uint64_t n_m, const double* maturity,
+                                    uint64_t n_tau, const double* vol, uint64_t n_vol, const double* rate,
+                                    uint64_t n_rate, double K_ref, double dividend_yield, MangoOptionType type,
+                                    const MangoGridConfig* pde_grid, double max_failure_rate,
+                                    MangoPriceTable** out, MangoTableBuildInfo* info, MangoError* err) {
+    if (!out) return fail(err, MANGO_ERR_VALIDATION, "null out");
+    *out = nullptr;
+    if ((!log_moneyness && n_m) || (!maturity && n_tau) || (!vol && n_vol) || (!rate && n_rate))
+        return fail(err, MANGO_ERR_VALIDATION, "null axis");
+    try {
+        std::array<std::vector<double>, kTableDims> axes;
+        axes[0].assign(log_moneyness, log_moneyness + n_m);
+        axes[1].assign(maturity, maturity + n_tau);
+        axes[2].assign(vol, vol + n_vol);
+        axes[3].assign(rate, rate + n_rate);
+        std::shared_ptr<TableData> t;
+        const MangoStatus st = build_table(std::move(axes), K_ref, dividend_yield, type, pde_grid, max_failure_rate, t, err);
+        if (st != MANGO_OK) return st;
+        if (info) *info = t->info;
+        *out = new MangoPriceTable{std::move(t)};
+        return MANGO_OK;
+    } catch (const std::exception& ex) {
+        return fail(err, MANGO_ERR_SOLVER, ex.what());
+    }
+}
+
+MangoStatus mango_make_price_table(const MangoIvFactoryConfig* cfg, MangoPriceTable** out, MangoError* err) {
+    if (!cfg || !out) return fail(err, MANGO_ERR_VALIDATION, "null cfg or out");
+    *out = nullptr;
+    try {
+        std::shared_ptr<TableData> t;
+        const MangoStatus st = factory_table(cfg, t, err);
+        if (st != MANGO_OK) return st;
+        *out = new MangoPriceTable{std::move(t)};
+        return MANGO_OK;
+    } catch (const std::exception& ex) {
+        return fail(err, MANGO_ERR_SOLVER, ex.what());
+    }
+}
+
+MangoStatus mango_make_interp_iv_solver(const MangoIvFactoryConfig* cfg, MangoInterpIvSolver** out, MangoError* err) {
+    if (!cfg || !out) return fail(err, MANGO_ERR_VALIDATION, "null cfg or out");
+    *out = nullptr;
+    try {
+        std::shared_ptr<TableData> t;
+        const MangoStatus st = factory_table(cfg, t, err);
+        if (st != MANGO_OK) return st;
+        return make_solver(t, &cfg->solver_config, out, err);
+    } catch (const std::exception& ex) {
+        return fail(err, MANGO_ERR_SOLVER, ex.what());
+    }
+}
+
+MangoStatus mango_price_table_validate(const MangoPriceTable* t, const MangoPricingParams* p, MangoError* err) {
+    if (!t || !p) return fail(err, MANGO_ERR_VALIDATION, "null table or params");
+    const char* why = nullptr;
+    const MangoStatus base = validate_params(*p, &why);
+    if (base != MANGO_OK) return fail(err, MANGO_ERR_VALIDATION, why ? why : "invalid pricing parameters");
+    const TableData& d = *t->t;
+    if (p->option_type != d.type) return fail(err, MANGO_ERR_VALIDATION, "option type does not match the table");
+    if (std::abs(p->dividend_yield - d.q) > 1e-10) return fail(err, MANGO_ERR_VALIDATION, "dividend yield does not match the table");
+    const double lm = std::log(p->spot / p->strike);
+    const double v[4] = {lm, p->maturity, p->volatility, p->rate_const};
+    static const char* names[4] = {"log-moneyness", "maturity", "volatility", "rate"};
+    for (int a = 0; a < kTableDims; ++a) {
+        const auto& g = d.axes[static_cast<size_t>(a)];
+        if (v[a] < g.front() || v[a] > g.back()) {
+            const std::string m = std::string(names[a]) + " outside the table range";
+            return fail(err, MANGO_ERR_VALIDATION, m);
+        }
+    }
+    return MANGO_OK;
+}
+
+double mango_price_table_price(const MangoPriceTable* t, const MangoPricingParams* p) { return price_one(t, p, false); }
+double mango_price_table_vega(const MangoPriceTable* t, const MangoPricingParams* p) { return price_one(t, p, true); }
+
+MangoOptionType mango_price_table_option_type(const MangoPriceTable* t) { return t ? t->t->type : MANGO_PUT; }
+double mango_price_table_dividend_yield(const MangoPriceTable* t) {
+    return t ? t->t->q : std::numeric_limits<double>::quiet_NaN();
+}
+
+MangoStatus mango_price_table_make_iv_solver(const MangoPriceTable* t, const MangoInterpSolverConfig* cfg,
+                                             MangoInterpIvSolver** out, MangoError* err) {
+    if (!t || !out) return fail(err, MANGO_ERR_VALIDATION, "null table or out");
+    *out = nullptr;
+    return make_solver(t->t, cfg, out, err);
+}
+
+void mango_price_table_free(MangoPriceTable* t) { delete t; }
+void mango_interp_iv_solver_free(MangoInterpIvSolver* s) { delete s; }
+
+MangoStatus mango_price_table_shape(const MangoPriceTable* t, uint64_t* dims, double* K_ref, MangoError* err) {
+    if (!t) return fail(err, MANGO_ERR_VALIDATION, "null table");
+    if (dims) for (int a = 0; a < kTableDims; ++a) dims[a] = t->t->axes[static_cast<size_t>(a)].size();
+    if (K_ref) *K_ref = t->t->K_ref;
+    return MANGO_OK;
+}
+
+MangoStatus mango_price_table_export(const MangoPriceTable* t, double* axes, double* coeffs, double* prices,
+                                     MangoError* err) {
+    if (!t) return fail(err, MANGO_ERR_VALIDATION, "null table");
+    const TableData& d = *t->t;
+    if (axes)
+        for (const auto& a : d.axes) { std::copy(a.begin(), a.end(), axes); axes += a.size(); }
+    if (coeffs && hipMemcpy(coeffs, d.view.coeffs, d.size * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(err, MANGO_ERR_SOLVER, "copy of the coefficients failed");
+    if (prices && hipMemcpy(prices, d.d_prices, d.size * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(err, MANGO_ERR_SOLVER, "copy of the extracted prices failed");
+    return MANGO_OK;
+}
+
+MangoStatus mango_price_table_price_batch(const MangoPriceTable* t, const MangoPricingParams* params, uint64_t n,
+                                          double* out_price, double* out_vega, MangoError* err) {
+    if (!t || (!params && n) || (!out_price && n)) return fail(err, MANGO_ERR_VALIDATION, "null table, params or output");
+    std::vector<double> rows(static_cast<size_t>(n) * 5, std::numeric_limits<double>::quiet_NaN());
+    std::vector<char> ok(static_cast<size_t>(n));
+    for (uint64_t i = 0; i < n; ++i) ok[i] = price_row(&params[i], rows.data() + 5 * i);
+    std::vector<double> pr, vg;
+    std::lock_guard<std::mutex> lk(g_query_mu);
+    const MangoStatus st = run_price(*t->t, rows, pr, out_vega ? &vg : nullptr, err);
+    if (st != MANGO_OK) return st;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    for (uint64_t i = 0; i < n; ++i) {
+        out_price[i] = ok[i] ? pr[i] : nan;
+        if (out_vega) out_vega[i] = ok[i] ? vg[i] : nan;
+    }
+    return MANGO_OK;
+}
+
+MangoStatus mango_price_table_price_device(const MangoPriceTable* t, const double* d_points, uint64_t n,
+                                           double* d_price, double* d_vega, void* stream, MangoError* err) {
+    if (!t || (n && (!d_points || !d_price))) return fail(err, MANGO_ERR_VALIDATION, "null table, points or output");
+    if (!device_ready(err)) return MANGO_ERR_SOLVER;
+    if (launch_table_price(t->t->view, d_points, static_cast<int64_t>(n), d_price, d_vega,
+                           static_cast<hipStream_t>(stream)) != 0)
+        return fail(err, MANGO_ERR_SOLVER, "table price launch failed");
+    return MANGO_OK;
+}
+
+MangoStatus mango_interp_iv_solve_device(const MangoInterpIvSolver* s, const MangoDeviceIvQuery* d_queries,
+                                         uint64_t n, MangoDeviceIvResult* d_out, void* stream, MangoError* err) {
+    if (!s || (n && (!d_queries || !d_out))) return fail(err, MANGO_ERR_VALIDATION, "null solver, queries or output");
+    if (!device_ready(err)) return MANGO_ERR_SOLVER;
+    if (launch_interp_iv(s->t->view, s->cfg, reinterpret_cast<const IvQueryRec*>(d_queries), static_cast<int64_t>(n),
+                         reinterpret_cast<IvResultRec*>(d_out), static_cast<hipStream_t>(stream)) != 0)
+        return fail(err, MANGO_ERR_SOLVER, "interpolated IV launch failed");
+    return MANGO_OK;
+}
+
+MangoStatus mango_interp_iv_solve_batch(const MangoInterpIvSolver* s, const MangoIvQuery* queries, uint64_t n,
+                                        MangoIvBatchSlot* out_slots, uint64_t* out_failed_count, MangoError* err) {
+    if (!s || (!queries && n > 0) || (!out_slots && n > 0))
+        return fail(err, MANGO_ERR_VALIDATION, "null solver, queries, or out_slots");
+    try {
+        std::vector<IvQueryRec> recs;
+        std::vector<uint64_t> where;
+        recs.reserve(n);
+        for (uint64_t i = 0; i < n; ++i) {
+            IvQueryRec r;
+            const char* why = nullptr;
+            if (to_query(queries[i], r, &why)) {
+                recs.push_back(r);
+                where.push_back(i);
+            } else {
+                out_slots[i].status = MANGO_ERR_VALIDATION;
+                out_slots[i].success = MangoIvSuccess{};
+            }
+        }
+        std::vector<IvResultRec> res;
+        const MangoStatus st = run_iv(s, recs, res, err);
+        if (st != MANGO_OK) return st;
+        for (size_t k = 0; k < where.size(); ++k) {
+            MangoIvBatchSlot& slot = out_slots[where[k]];
+            slot.status = res[k].status;
+            slot.success = MangoIvSuccess{};
+            if (res[k].status == MANGO_OK) {
+                slot.success.implied_vol = res[k].implied_vol;
+                slot.success.iterations = static_cast<uint64_t>(res[k].iterations);
+                slot.success.final_error = res[k].final_error;
+            }
+        }
+        if (out_failed_count) {
+            uint64_t failed = 0;
+            for (uint64_t i = 0; i < n; ++i) failed += out_slots[i].status != MANGO_OK;
+            *out_failed_count = failed;
+        }
+        return MANGO_OK;
+    } catch (const std::exception& ex) {
+        return fail(err, MANGO_ERR_SOLVER, ex.what());
+    }
+}
+
+MangoStatus mango_interp_iv_solve(const MangoInterpIvSolver* s, const MangoIvQuery* q, MangoIvSuccess* out,
+                                  MangoError* err) {
+    if (!s || !q || !out) return fail(err, MANGO_ERR_VALIDATION, "null solver, query, or out");
+    IvQueryRec r;
+    const char* why = nullptr;
+    if (!to_query(*q, r, &why)) return fail(err, MANGO_ERR_VALIDATION, why);
+    std::vector<IvResultRec> res;
+    const MangoStatus st = run_iv(s, std::vector<IvQueryRec>{r}, res, err);
+    if (st != MANGO_OK) return st;
+    if (res[0].status != MANGO_OK) return fail(err, res[0].status, iv_status_message(res[0].status));
+    *out = MangoIvSuccess{};
+    out->implied_vol = res[0].implied_vol;
+    out->iterations = static_cast<uint64_t>(res[0].iterations);
+    out->final_error = res[0].final_error;
+    return MANGO_OK;
+}
+
+}  // extern "C"

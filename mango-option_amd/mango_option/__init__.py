@@ -412,3 +412,12 @@ class BatchAmericanOptionSolver:
                     else SolverErrorCode.LinearSolveFailure
                 results.append((False, None, SolverError(code)))
         return results, failed
+
+
+from .table import (BSplineBackend, InterpolatedIVSolver, InterpolatedIVSolverConfig, IVError, IVGrid,  # noqa: E402
+                    IVQuery, IVSolverFactoryConfig, IVSuccess, PriceTable, PriceTableBuilder, PriceTableConfig,
+                    make_interpolated_iv_solver, make_price_table)
+
+__all__ += ["BSplineBackend", "InterpolatedIVSolver", "InterpolatedIVSolverConfig", "IVError", "IVGrid", "IVQuery",
+            "IVSolverFactoryConfig", "IVSuccess", "PriceTable", "PriceTableBuilder", "PriceTableConfig",
+            "make_interpolated_iv_solver", "make_price_table"]

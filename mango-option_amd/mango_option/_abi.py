@@ -129,6 +129,99 @@ class MangoDeviceOption(C.Structure):
     ]
 
 
+class MangoIvQuery(C.Structure):
+    _fields_ = [
+        ("spot", C.c_double),
+        ("strike", C.c_double),
+        ("maturity", C.c_double),
+        ("dividend_yield", C.c_double),
+        ("market_price", C.c_double),
+        ("rate_const", C.c_double),
+        ("tenor_points", C.POINTER(MangoTenorPoint)),
+        ("n_tenor_points", C.c_uint64),
+        ("dividends", C.POINTER(MangoDividend)),
+        ("n_dividends", C.c_uint64),
+        ("option_type", C.c_int32),
+    ]
+
+
+class MangoIvSuccess(C.Structure):
+    _fields_ = [("implied_vol", C.c_double), ("iterations", C.c_uint64), ("final_error", C.c_double),
+                ("vega", C.c_double), ("has_vega", C.c_int32), ("used_rate_approximation", C.c_int32)]
+
+
+class MangoInterpSolverConfig(C.Structure):
+    _fields_ = [("max_iter", C.c_uint64), ("tolerance", C.c_double), ("sigma_min", C.c_double),
+                ("sigma_max", C.c_double), ("vega_threshold", C.c_double)]
+
+
+class MangoAdaptiveGridParams(C.Structure):
+    _fields_ = [("target_iv_error", C.c_double), ("max_iter", C.c_uint64), ("max_points_per_dim", C.c_uint64),
+                ("min_moneyness_points", C.c_uint64), ("validation_samples", C.c_uint64),
+                ("refinement_factor", C.c_double), ("lhs_seed", C.c_uint64), ("vega_floor", C.c_double),
+                ("max_failure_rate", C.c_double)]
+
+
+class MangoMultiKRef(C.Structure):
+    _fields_ = [("K_refs", C.POINTER(C.c_double)), ("n_K_refs", C.c_uint64), ("K_ref_count", C.c_int32),
+                ("K_ref_span", C.c_double)]
+
+
+class MangoDiscreteDividendConfig(C.Structure):
+    _fields_ = [("maturity", C.c_double), ("dividends", C.POINTER(MangoDividend)), ("n_dividends", C.c_uint64),
+                ("kref_config", MangoMultiKRef)]
+
+
+class MangoIvFactoryConfig(C.Structure):
+    _fields_ = [
+        ("option_type", C.c_int32),
+        ("spot", C.c_double),
+        ("dividend_yield", C.c_double),
+        ("moneyness", C.POINTER(C.c_double)),
+        ("n_moneyness", C.c_uint64),
+        ("vol", C.POINTER(C.c_double)),
+        ("n_vol", C.c_uint64),
+        ("rate", C.POINTER(C.c_double)),
+        ("n_rate", C.c_uint64),
+        ("maturity_grid", C.POINTER(C.c_double)),
+        ("n_maturity", C.c_uint64),
+        ("solver_config", MangoInterpSolverConfig),
+        ("adaptive", C.POINTER(MangoAdaptiveGridParams)),
+        ("discrete_dividends", C.POINTER(MangoDiscreteDividendConfig)),
+    ]
+
+
+class MangoIvBatchSlot(C.Structure):
+    _fields_ = [("status", C.c_int32), ("success", MangoIvSuccess)]
+
+
+class MangoTableBuildInfo(C.Structure):
+    _fields_ = [("n_pde_solves", C.c_uint64), ("failed_pde", C.c_uint64), ("repaired_full_slices", C.c_uint64),
+                ("repaired_partial_points", C.c_uint64), ("fallback_grid", C.c_int32), ("reserved", C.c_int32),
+                ("max_fit_residual", C.c_double), ("build_ms", C.c_double)]
+
+
+class MangoDeviceIvQuery(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("spot", "strike", "maturity", "rate", "dividend_yield", "market_price")] + \
+               [("option_type", C.c_int32), ("reserved", C.c_int32)]
+
+
+class MangoDeviceIvResult(C.Structure):
+    _fields_ = [("implied_vol", C.c_double), ("final_error", C.c_double), ("iterations", C.c_int32),
+                ("status", C.c_int32)]
+
+
+assert C.sizeof(MangoIvQuery) == 88 and MangoIvQuery.market_price.offset == 32
+assert C.sizeof(MangoIvSuccess) == 40
+assert C.sizeof(MangoInterpSolverConfig) == 40
+assert C.sizeof(MangoAdaptiveGridParams) == 72
+assert C.sizeof(MangoMultiKRef) == 32
+assert C.sizeof(MangoDiscreteDividendConfig) == 56
+assert C.sizeof(MangoIvFactoryConfig) == 144 and MangoIvFactoryConfig.solver_config.offset == 88
+assert C.sizeof(MangoIvBatchSlot) == 48
+assert C.sizeof(MangoTableBuildInfo) == 56
+assert C.sizeof(MangoDeviceIvQuery) == 56
+assert C.sizeof(MangoDeviceIvResult) == 24
 assert C.sizeof(MangoPricingParams) == 88
 assert MangoPricingParams.rate_const.offset == 40
 assert MangoPricingParams.option_type.offset == 80
@@ -190,6 +283,33 @@ SIGNATURES = {
     "mango_tuning_variants": (C.c_uint64, [_P(MangoVariantRule), C.c_uint64]),
     "mango_profile_enable": (None, [C.c_int32]),
     "mango_profile_read": (C.c_int32, [_P(C.c_double), _P(C.c_uint64)]),
+    "mango_make_price_table": (C.c_int32, [_P(MangoIvFactoryConfig), _P(C.c_void_p), _P(MangoError)]),
+    "mango_price_table_validate": (C.c_int32, [C.c_void_p, _P(MangoPricingParams), _P(MangoError)]),
+    "mango_price_table_price": (C.c_double, [C.c_void_p, _P(MangoPricingParams)]),
+    "mango_price_table_vega": (C.c_double, [C.c_void_p, _P(MangoPricingParams)]),
+    "mango_price_table_option_type": (C.c_int32, [C.c_void_p]),
+    "mango_price_table_dividend_yield": (C.c_double, [C.c_void_p]),
+    "mango_price_table_make_iv_solver": (C.c_int32, [C.c_void_p, _P(MangoInterpSolverConfig), _P(C.c_void_p),
+                                                     _P(MangoError)]),
+    "mango_price_table_free": (None, [C.c_void_p]),
+    "mango_make_interp_iv_solver": (C.c_int32, [_P(MangoIvFactoryConfig), _P(C.c_void_p), _P(MangoError)]),
+    "mango_interp_iv_solve": (C.c_int32, [C.c_void_p, _P(MangoIvQuery), _P(MangoIvSuccess), _P(MangoError)]),
+    "mango_interp_iv_solve_batch": (C.c_int32, [C.c_void_p, _P(MangoIvQuery), C.c_uint64, _P(MangoIvBatchSlot),
+                                                _P(C.c_uint64), _P(MangoError)]),
+    "mango_interp_iv_solver_free": (None, [C.c_void_p]),
+    "mango_price_table_build": (C.c_int32, [_P(C.c_double), C.c_uint64, _P(C.c_double), C.c_uint64,
+                                            _P(C.c_double), C.c_uint64, _P(C.c_double), C.c_uint64, C.c_double,
+                                            C.c_double, C.c_int32, _P(MangoGridConfig), C.c_double,
+                                            _P(C.c_void_p), _P(MangoTableBuildInfo), _P(MangoError)]),
+    "mango_price_table_shape": (C.c_int32, [C.c_void_p, _P(C.c_uint64), _P(C.c_double), _P(MangoError)]),
+    "mango_price_table_export": (C.c_int32, [C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_double),
+                                             _P(MangoError)]),
+    "mango_price_table_price_batch": (C.c_int32, [C.c_void_p, _P(MangoPricingParams), C.c_uint64, _P(C.c_double),
+                                                  _P(C.c_double), _P(MangoError)]),
+    "mango_price_table_price_device": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                                   C.c_void_p, _P(MangoError)]),
+    "mango_interp_iv_solve_device": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                                 _P(MangoError)]),
 }
 
 _lib = None

@@ -492,7 +492,8 @@ template <typename F>
 inline int32_t brent(F&& f, double a, double b, double tol, size_t max_iter, double& root, size_t& iters, double& err) {
     double fa = f(a), fb = f(b);
     iters = 0;
-    if (!std::isfinite(fa) || !std::isfinite(fb)) return 5;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    if (!std::isfinite(fa) || !std::isfinite(fb)) { err = nan; return 5; }
     if (fa * fb > 0.0) { err = std::min(std::abs(fa), std::abs(fb)); return 4; }
     if (std::abs(fa) < tol) { root = a; err = std::abs(fa); return 0; }
     if (std::abs(fb) < tol) { root = b; err = std::abs(fb); return 0; }
@@ -515,7 +516,7 @@ inline int32_t brent(F&& f, double a, double b, double tol, size_t max_iter, dou
         if (c1 || c2 || c3 || c4 || c5) { s = (a + b) / 2.0; mflag = true; }
         else mflag = false;
         const double fs = f(s);
-        if (!std::isfinite(fs)) { iters = it + 1; return 5; }
+        if (!std::isfinite(fs)) { iters = it + 1; err = nan; root = s; return 5; }
         d = c; c = b; fc = fb;
         if (fa * fs < 0.0) { b = s; fb = fs; } else { a = s; fa = fs; }
         if (std::abs(fa) < std::abs(fb)) { std::swap(a, b); std::swap(fa, fb); }

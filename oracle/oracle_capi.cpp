@@ -306,6 +306,23 @@ int32_t oracle_table_build(const double* lm, uint64_t nm, const double* tau, uin
 
 double oracle_eep_floor(double x) { return eep_floor(x); }
 
+// The EEP transform of build_price_table applied to a given tensor of
+// normalized American prices [n0][n1][n2][n3] (V / K_ref): out = eep_floor(
+// K_ref v - European(e^m K_ref, K_ref, tau, sigma, r, q)).
+void oracle_eep_tensor(const double* a0, uint64_t n0, const double* a1, uint64_t n1, const double* a2, uint64_t n2,
+                       const double* a3, uint64_t n3, double K_ref, double q, int32_t type, const double* v,
+                       double* out) {
+    const OptionType t = type == MANGO_PUT ? OptionType::PUT : OptionType::CALL;
+    for (uint64_t i = 0; i < n0; ++i)
+        for (uint64_t j = 0; j < n1; ++j)
+            for (uint64_t s = 0; s < n2; ++s)
+                for (uint64_t r = 0; r < n3; ++r) {
+                    const uint64_t k = ((i * n1 + j) * n2 + s) * n3 + r;
+                    const double eu = european_price(std::exp(a0[i]) * K_ref, K_ref, a1[j], a2[s], a3[r], q, t);
+                    out[k] = eep_floor(K_ref * v[k] - eu);
+                }
+}
+
 // Separable 4-D fit in place (values -> coefficients); returns 0 on success.
 int32_t oracle_bspline_fit4(const double* a0, uint64_t n0, const double* a1, uint64_t n1, const double* a2,
                             uint64_t n2, const double* a3, uint64_t n3, double* values, double* out_max_residual) {

@@ -59,6 +59,8 @@ def load() -> C.CDLL:
                                            P(MangoGridAccuracy), _dp, _dp, P(C.c_uint64), P(C.c_int32), _dp,
                                            C.c_int32]),
         "oracle_eep_floor": (C.c_double, [C.c_double]),
+        "oracle_eep_tensor": (None, [_dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, C.c_double,
+                                     C.c_double, C.c_int32, _dp, _dp]),
         "oracle_bspline_fit4": (C.c_int32, [_dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, _dp]),
         "oracle_bspline_eval4": (None, [_dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, _dp, C.c_uint64, C.c_int32, _dp]),
         "oracle_iv_batch": (None, [_dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_double, C.c_double, C.c_int32, _dp, P(C.c_int32), C.c_uint64,
@@ -302,3 +304,12 @@ def bspline_eval4(axes, coeffs, points, partial_axis=-1):
 
 def eep_floor(x: float) -> float:
     return load().oracle_eep_floor(float(x))
+
+
+def eep_tensor(axes, K_ref, q, put, prices):
+    """EEP transform of a normalized price tensor, as the table build applies it."""
+    axes = [_a(a) for a in axes]
+    v = _a(prices).reshape(-1)
+    out = np.empty_like(v)
+    load().oracle_eep_tensor(*_axes_args(axes), float(K_ref), float(q), 1 if put else 0, _p(v), _p(out))
+    return out.reshape([len(a) for a in axes])

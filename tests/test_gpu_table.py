@@ -1,0 +1,225 @@
+# SPDX-License-Identifier: MIT
+"""GPU tests of the device price-table build and the interpolated-IV batch
+against the oracle's restatement (pinned in test_oracle_table.py by cubic
+reproduction, node interpolation, snapshot splines and IV round trips).
+
+Parity is checked stage by stage, each stage fed the same inputs:
+  * extraction: the snapshot solves' prices V/K_ref at the table nodes within
+    1e-11 (1e-9 in dollars at K_ref = 100), the solve kernels' price bar;
+  * fit: the oracle's separable fit of the device's extracted prices equals
+    the device coefficients within 1e-9 of the coefficient scale;
+  * evaluation: price and vega from the device coefficients within 1e-9 of
+    the oracle's evaluation of the same coefficients;
+  * interpolated IV on the same coefficients: identical statuses and Brent
+    iteration counts, implied volatilities within 1e-9.
+End to end (device table against the oracle's own table) the extraction
+difference passes through the fit, whose collocation inverse amplifies it by
+a bounded factor; prices there are held to 1e-8 plus 1e-10 of the sum of
+|coefficient| x basis weight at the point (large where the table
+extrapolates), and implied volatilities to 1e-9 with the same statuses.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import mango_option as M
+from mango_option import _abi, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+PRICE_ABS = 1e-9
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _explicit(n_points, n_time, x_min=-3.0, x_max=3.0):
+    spec = M.GridSpec(x_min=x_min, x_max=x_max, n_points=n_points)
+    return M.grid_config(_abi.MANGO_GRID_MODE_EXPLICIT, spec=spec, n_time=n_time), spec.to_c()
+
+
+@pytest.fixture(scope="module")
+def c4():
+    _say("table: C4 build (201-point sinh grid, 1000 steps, 50 x 30 x 20 x 10)")
+    ax = W.c4_axes()
+    cfg, spec = _explicit(201, 1000)
+    t = M.PriceTableBuilder.from_vectors(*ax, W.C4_K_REF, cfg, M.OptionType.PUT, W.C4_DIVIDEND).build()
+    ref = O.table_build(ax, W.C4_K_REF, W.C4_DIVIDEND, True, spec=spec, n_time=1000, n_threads=8)
+    return t, ref
+
+
+def test_c4_build_info(c4):
+    t, ref = c4
+    info = t.build_info
+    assert info.fallback_grid == 1 and ref.fallback
+    assert info.n_pde_solves == 200 == ref.pde_solves
+    assert info.failed_pde == 0 and info.repaired_full_slices == 0
+    assert info.repaired_partial_points > 0  # high-sigma solves record 29 distinct steps for 30 maturities
+    assert info.max_fit_residual < 1e-9
+    _say(f"table: build {info.build_ms:.2f} ms, partial repairs {info.repaired_partial_points}")
+
+
+def _abs_sum(ax, coeffs, rows):
+    """sum |c| w of the tensor-product evaluation at rows (S, K, tau, sigma, r)."""
+    pts = np.stack([np.log(rows[:, 0] / rows[:, 1]), rows[:, 2], rows[:, 3], rows[:, 4]], 1)
+    return O.bspline_eval4(ax, np.abs(coeffs), pts) * rows[:, 1] / W.C4_K_REF
+
+
+def test_c4_extracted_prices(c4):
+    t, ref = c4
+    axes, coeffs, prices = t.export()
+    for a, b in zip(axes, ref.axes):
+        assert np.array_equal(a, b)
+    d = np.abs(prices - ref.tensor)
+    _say(f"table: extracted V/K max |diff| {d.max():.3e}")
+    assert d.max() <= PRICE_ABS / W.C4_K_REF
+
+
+def test_c4_fit_of_same_prices(c4):
+    t, ref = c4
+    axes, coeffs, prices = t.export()
+    eep = O.eep_tensor(axes, W.C4_K_REF, W.C4_DIVIDEND, True, prices)
+    want, res = O.bspline_fit4(axes, eep)
+    scale = np.abs(want).max()
+    dc = np.abs(coeffs - want).max()
+    _say(f"table: coefficient max |diff| {dc:.3e} (scale {scale:.3e})")
+    assert dc <= 1e-9 * max(scale, 1.0)
+
+
+def _rows(n, seed):
+    pts, _ = W.c4_query_points(n, seed=seed)
+    S = W.C4_K_REF * np.exp(pts[:, 0])
+    rows = np.stack([S, np.full(n, W.C4_K_REF), pts[:, 1], pts[:, 2], pts[:, 3]], 1)
+    # points outside the axes (clamped) and at the far node
+    return np.concatenate([rows, [[300.0, 100.0, 5.0, 1.5, 0.2], [20.0, 100.0, 0.001, 0.01, -0.1],
+                                  [100.0, 100.0, 2.0, 0.8, 0.1]]])
+
+
+def test_c4_price_and_vega(c4):
+    t, ref = c4
+    axes, coeffs, _ = t.export()
+    rows = _rows(4096, 31)
+    pr, vg = t.price_points(rows)
+    same = O.Table(axes, coeffs, W.C4_K_REF, W.C4_DIVIDEND, True)
+    rp, rv = same.price(rows)
+    assert np.max(np.abs(pr - rp)) <= PRICE_ABS
+    assert np.max(np.abs(vg - rv)) <= PRICE_ABS
+    # end to end against the oracle's own table
+    ep, _ = ref.price(rows)
+    bound = 1e-8 + 1e-10 * _abs_sum(axes, ref.coeffs, rows)
+    _say(f"table: end-to-end price max |diff| {np.max(np.abs(pr - ep)):.3e}")
+    assert np.all(np.abs(pr - ep) <= bound)
+    # scalar C entry points
+    p = M.PricingParams(spot=95.0, strike=100.0, maturity=0.7, rate=0.04, dividend_yield=W.C4_DIVIDEND,
+                        option_type=M.OptionType.PUT, volatility=0.3)
+    a, v = same.price(np.array([[95.0, 100.0, 0.7, 0.3, 0.04]]))
+    assert abs(t.price(p) - a[0]) <= PRICE_ABS
+    assert abs(t.vega(p) - v[0]) <= PRICE_ABS
+
+
+def _queries(t, n, seed):
+    pts, noise = W.c4_query_points(n, seed=seed)
+    S = W.C4_K_REF * np.exp(pts[:, 0])
+    K = np.full(n, W.C4_K_REF)
+    pr, _ = t.price_points(np.stack([S, K, pts[:, 1], pts[:, 2], pts[:, 3]], 1))
+    return S, K, pts[:, 1], pts[:, 3], pr * noise
+
+
+def test_c4_interpolated_iv_matches_oracle(c4):
+    t, ref = c4
+    axes, coeffs, _ = t.export()
+    S, K, tau, r, market = _queries(t, 20000, seed=4)
+    solver = t.make_iv_solver()
+    iv, it, err, st = solver.solve_arrays(S, K, tau, r, np.full(len(S), W.C4_DIVIDEND), market)
+    rows = np.stack([S, K, tau, r, np.full(len(S), W.C4_DIVIDEND), market], 1)
+    # same coefficients: same decisions, same Brent path
+    same = O.Table(axes, coeffs, W.C4_K_REF, W.C4_DIVIDEND, True)
+    siv, sit, _, sst = same.iv(rows, n_threads=8)
+    _say(f"iv: statuses {np.bincount(st, minlength=6)}")
+    assert np.array_equal(st, sst)
+    ok = st == 0
+    assert ok.mean() > 0.5
+    assert np.max(np.abs(iv[ok] - siv[ok])) <= 1e-9
+    assert np.array_equal(it[ok], sit[ok].astype(np.int32))
+    assert np.all(np.isnan(iv[~ok]))
+    # end to end against the oracle's own table
+    riv, _, _, rst = ref.iv(rows, n_threads=8)
+    assert np.array_equal(st, rst)
+    assert np.max(np.abs(iv[ok] - riv[ok])) <= 1e-9
+
+
+def test_c_abi_batch_and_single_queries(c4):
+    t, ref = c4
+    S, K, tau, r, market = _queries(t, 64, seed=41)
+    qs = [M.IVQuery(spot=S[i], strike=K[i], maturity=tau[i], rate=r[i], dividend_yield=W.C4_DIVIDEND,
+                    option_type=M.OptionType.PUT, market_price=market[i]) for i in range(64)]
+    # statuses the boundary decides: type mismatch, dividend mismatch, no bracket, arbitrage
+    qs += [M.IVQuery(100.0, 100.0, 1.0, 0.05, W.C4_DIVIDEND, M.OptionType.CALL, 10.0),
+           M.IVQuery(100.0, 100.0, 1.0, 0.05, 0.03, M.OptionType.PUT, 10.0),
+           M.IVQuery(100.0, 100.0, 1.0, 0.05, W.C4_DIVIDEND, M.OptionType.PUT, 40.0),
+           M.IVQuery(80.0, 100.0, 1.0, 0.05, W.C4_DIVIDEND, M.OptionType.PUT, 19.0)]
+    solver = t.make_iv_solver()
+    res, failed = solver.solve_batch(qs)
+    rows = np.array([[q.spot, q.strike, q.maturity, q.rate, q.dividend_yield, q.market_price] for q in qs])
+    puts = np.array([q.option_type == M.OptionType.PUT for q in qs])
+    riv = np.full(len(qs), np.nan)
+    rst = np.zeros(len(qs), dtype=np.int32)
+    for flag in (True, False):
+        sel = puts == flag
+        a, _, _, s = ref.iv(rows[sel], put=flag)
+        riv[sel], rst[sel] = a, s
+    st = np.array([0 if ok else e.code for ok, _, e in res])
+    assert np.array_equal(st, rst)
+    assert list(st[-4:]) == [1, 1, 4, 2]
+    assert failed == int(np.sum(rst != 0))
+    for i, (ok, s, _) in enumerate(res):
+        if ok:
+            assert abs(s.implied_vol - riv[i]) <= 1e-9
+    ok, s, e = solver.solve(qs[0])
+    assert ok == (rst[0] == 0)
+    ok, s, e = solver.solve(qs[-1])
+    assert not ok and e.code == 2
+
+
+def test_explicit_grid_without_fallback():
+    ax = (np.linspace(-0.5, 0.5, 12), np.linspace(0.1, 1.0, 8), np.linspace(0.1, 0.3, 6), np.linspace(0.0, 0.06, 4))
+    cfg, spec = _explicit(301, 400, -2.5, 2.5)
+    _say("table: explicit 301 x 400 grid (no fallback), calls")
+    t = M.PriceTableBuilder.from_vectors(*ax, 100.0, cfg, M.OptionType.CALL, 0.01).build()
+    ref = O.table_build(ax, 100.0, 0.01, False, spec=spec, n_time=400, n_threads=8)
+    assert t.build_info.fallback_grid == 0 and not ref.fallback
+    axes, coeffs, prices = t.export()
+    assert np.max(np.abs(prices - ref.tensor)) <= PRICE_ABS / 100.0
+    want, _ = O.bspline_fit4(axes, O.eep_tensor(axes, 100.0, 0.01, False, prices))
+    assert np.abs(coeffs - want).max() <= 1e-9 * max(np.abs(want).max(), 1.0)
+    pts = np.stack([np.random.default_rng(5).uniform(a[0], a[-1], 500) for a in ax], 1)
+    rows = np.stack([100.0 * np.exp(pts[:, 0]), np.full(500, 100.0), pts[:, 1], pts[:, 2], pts[:, 3]], 1)
+    pr, _ = t.price_points(rows)
+    rp, _ = ref.price(rows)
+    assert np.max(np.abs(pr - rp)) <= 1e-8
+
+
+def test_factory_table_accuracy_grid():
+    cfg = M.IVSolverFactoryConfig(option_type=M.OptionType.PUT, spot=100.0, dividend_yield=0.01,
+                                  grid=M.IVGrid(moneyness=[0.7, 0.8, 0.9, 1.0, 1.1, 1.2, 1.3],
+                                                vol=[0.15, 0.2, 0.3, 0.4, 0.5],
+                                                rate=[0.0, 0.02, 0.04, 0.06]),
+                                  backend=M.BSplineBackend(maturity_grid=[0.1, 0.25, 0.5, 1.0, 1.5]))
+    _say("table: factory path (default accuracy grid)")
+    t = M.make_price_table(cfg)
+    ax = (np.log(cfg.grid.moneyness), cfg.backend.maturity_grid, cfg.grid.vol, cfg.grid.rate)
+    ref = O.table_build(ax, 100.0, 0.01, True, n_threads=8)
+    _, coeffs, prices = t.export()
+    assert np.max(np.abs(prices - ref.tensor)) <= PRICE_ABS / 100.0
+    assert t.option_type == M.OptionType.PUT and t.dividend_yield == 0.01
+    s = M.make_interpolated_iv_solver(cfg)
+    ok, res, _ = s.solve(M.IVQuery(100.0, 100.0, 0.5, 0.03, 0.01, M.OptionType.PUT, 7.0))
+    a, _, _, st = ref.iv(np.array([[100.0, 100.0, 0.5, 0.03, 0.01, 7.0]]))
+    assert ok and st[0] == 0 and abs(res.implied_vol - a[0]) <= 1e-9
+    with pytest.raises(ValueError):
+        M.make_price_table(M.IVSolverFactoryConfig(grid=M.IVGrid(moneyness=[0.9, 1.0, 1.1], vol=[0.2] * 4,
+                                                                 rate=[0.0] * 4),
+                                                   backend=M.BSplineBackend(maturity_grid=[0.5] * 4)))
