@@ -79,6 +79,10 @@ int max_nodes_supported();
 // Variant choice: the first rule with n_space <= max_n, else the built-in default.
 void set_variant_rules(const int64_t* max_n, const SolveVariant* v, int n);
 SolveVariant pick_variant(int64_t n_space);
+// Variant for launches too small to fill the chip (latency-bound): the
+// tuning rules when set, else one option per wavefront.
+SolveVariant pick_latency_variant(int64_t n_space);
+constexpr int kLatencyJobs = 1024;  // SIMDs of an MI355X (256 CUs x 4)
 
 // Per-option epilogue of the normalized chain: Greeks of option i from the
 // stored state of PDE group group_of[i].

@@ -276,9 +276,14 @@ MangoStatus run_jobs(std::vector<PdeJob>& jobs, size_t n_out, Greeks* greeks, in
     sorted.reserve(jobs.size());
     std::vector<std::pair<SolveVariant, std::pair<size_t, size_t>>> ranges;
     {
-        std::vector<SolveVariant> var(jobs.size());
-        for (size_t i = 0; i < jobs.size(); ++i) var[i] = pick_variant(std::max<int64_t>(jobs[i].grid.n_points, 3));
-        const SolveVariant all[] = {{32, 4}, {64, 4}, {64, 7}, {64, 2}, {16, 8}};
+        std::vector<SolveVariant> var(jobs.size()), all;
+        for (size_t i = 0; i < jobs.size(); ++i) {
+            const int64_t nsp = std::max<int64_t>(jobs[i].grid.n_points, 3);
+            var[i] = jobs.size() <= static_cast<size_t>(kLatencyJobs) ? pick_latency_variant(nsp) : pick_variant(nsp);
+            if (std::none_of(all.begin(), all.end(), [&](const SolveVariant& a) {
+                    return a.lanes == var[i].lanes && a.nodes == var[i].nodes; }))
+                all.push_back(var[i]);
+        }
         for (const auto& v : all) {
             const size_t start = sorted.size();
             for (size_t i = 0; i < jobs.size(); ++i)
@@ -373,11 +378,19 @@ MangoStatus run_chain(std::vector<PdeJob>& gjobs, const std::vector<ChainOption>
     std::vector<PdeJob> sorted;
     std::vector<std::pair<SolveVariant, std::pair<size_t, size_t>>> ranges;
     {
-        const SolveVariant all[] = {{32, 4}, {64, 4}, {64, 7}};
+        std::vector<SolveVariant> all;
+        const bool few = gjobs.size() <= static_cast<size_t>(kLatencyJobs);
+        auto pick = [&](int64_t nsp) { return few ? pick_latency_variant(nsp) : pick_variant(nsp); };
+        for (const auto& j : gjobs) {
+            const SolveVariant pv = pick(j.grid.n_points);
+            if (std::none_of(all.begin(), all.end(), [&](const SolveVariant& a) {
+                    return a.lanes == pv.lanes && a.nodes == pv.nodes; }))
+                all.push_back(pv);
+        }
         for (const auto& v : all) {
             const size_t start = sorted.size();
             for (const auto& j : gjobs) {
-                const SolveVariant pv = pick_variant(j.grid.n_points);
+                const SolveVariant pv = pick(j.grid.n_points);
                 if (pv.lanes == v.lanes && pv.nodes == v.nodes) sorted.push_back(j);
             }
             if (sorted.size() > start) ranges.push_back({v, {start, sorted.size() - start}});
@@ -472,7 +485,8 @@ MangoStatus device_batch(const MangoDeviceOption* d_opts, uint64_t n, const Mang
     if (!d_jobs) return MANGO_ERR_SOLVER;
     if (launch_plan_jobs(reinterpret_cast<const DeviceOptionRecord*>(d_opts), static_cast<int>(n), plan, d_jobs,
                          stream) != 0 ||
-        timed_solve(pick_variant(n_max), d_jobs, static_cast<int>(n), reinterpret_cast<Greeks*>(d_out), d_status,
+        timed_solve(n <= static_cast<uint64_t>(kLatencyJobs) ? pick_latency_variant(n_max) : pick_variant(n_max),
+                    d_jobs, static_cast<int>(n), reinterpret_cast<Greeks*>(d_out), d_status,
                     StateOut{}, stream, nullptr) != 0) {
         set_err(err, MANGO_ERR_SOLVER, (std::string("HIP launch error: ") + hipGetErrorString(hipGetLastError())).c_str());
         return MANGO_ERR_SOLVER;
@@ -588,8 +602,10 @@ MangoStatus device_chains(const MangoDeviceOption* d_opts, uint64_t n, const int
     ok = ok && launch_plan_chains(reinterpret_cast<const DeviceOptionRecord*>(d_opts), d_seg_offsets,
                                   static_cast<int>(n_segments), plan, g_chain.jobs, g_chain.job_count, g_chain.copts,
                                   g_chain.rep, stream) == 0;
-    ok = ok && timed_solve(pick_variant(n_max), g_chain.jobs, ni, nullptr, g_chain.gstatus, so, stream,
-                           g_chain.job_count) == 0;
+    // about two PDEs (one per option type) per chain segment
+    const SolveVariant var = 2 * static_cast<int64_t>(n_segments) <= kLatencyJobs ? pick_latency_variant(n_max)
+                                                                                   : pick_variant(n_max);
+    ok = ok && timed_solve(var, g_chain.jobs, ni, nullptr, g_chain.gstatus, so, stream, g_chain.job_count) == 0;
     ok = ok && launch_chain_epilogue(g_chain.copts, ni, so, g_chain.gstatus, reinterpret_cast<Greeks*>(d_out),
                                      d_status, stream) == 0;
     if (!ok) {

@@ -57,3 +57,41 @@ def test_gpu_matches_c1_fixture():
     assert failed == 0
     assert np.max(np.abs(g[:, 0] - want[:, 0])) <= 1e-9
     np.testing.assert_allclose(g[:, 1:], want[:, 1:], rtol=1e-7, atol=1e-10)
+
+
+def _c4():
+    d = np.load(os.path.join(HERE, "c4_slice.npz"), allow_pickle=False)
+    return {k: d[k] for k in d.files}
+
+
+def test_oracle_reproduces_c4_slice_fixture():
+    d = _c4()
+    ax = W.c4_axes()
+    si, ri = int(d["sigma_index"]), int(d["rate_index"])
+    spec = M.GridSpec(int(d["grid_type"]), float(d["x_min"]), float(d["x_max"]), int(d["n_points"]),
+                      clusters=[tuple(c) for c in d["clusters"]]).to_c()
+    if int(d["grid_type"]) == _abi.MANGO_GRID_SINH:
+        spec.concentration = float(d["clusters"][0][1])
+    np.testing.assert_array_equal(O.generate(spec), d["x"])
+    p = O.one_param(1.0, 1.0, ax[1][-1], ax[2][si], ax[3][ri], W.C4_DIVIDEND)
+    st, snaps, times = O.solve_snapshots(p, spec, int(d["n_time"]), ax[1])
+    assert st == 0
+    np.testing.assert_array_equal(times, d["snapshot_times"])
+    np.testing.assert_allclose(snaps, d["snapshots"], rtol=0, atol=1e-14)
+    # the tensor columns: snapshot j of the deduplicated list, the last one repeated
+    for j in range(len(ax[1])):
+        k = min(j, len(snaps) - 1)
+        ok, v, _ = O.spline_eval(d["x"], snaps[k], ax[0])
+        assert ok
+        np.testing.assert_allclose(v, d["prices"][:, j], rtol=0, atol=1e-14)
+
+
+@pytest.mark.gpu
+def test_gpu_table_matches_c4_slice_fixture():
+    d = _c4()
+    ax = W.c4_axes()
+    cfg = M.grid_config(_abi.MANGO_GRID_MODE_EXPLICIT, spec=M.GridSpec(n_points=201), n_time=1000)
+    t = M.PriceTableBuilder.from_vectors(*ax, W.C4_K_REF, cfg, M.OptionType.PUT, W.C4_DIVIDEND).build()
+    _, _, prices = t.export()
+    si, ri = int(d["sigma_index"]), int(d["rate_index"])
+    assert np.max(np.abs(prices[:, :, si, ri] - d["prices"])) <= 1e-11

@@ -91,7 +91,9 @@ def test_adjoint_forward_equals_solve():
     torch.cuda.synchronize()
     g = d_g.cpu().numpy()
     assert (st == 0).all() and (d_s.cpu().numpy() == 0).all()
-    np.testing.assert_allclose(out[:, :4], g, rtol=1e-12, atol=1e-12)
+    # the two kernels may split an option over different lane counts (scan
+    # association differs): agreement to the solve kernels' parity bar
+    np.testing.assert_allclose(out[:, :4], g, rtol=1e-9, atol=1e-10)
 
 
 def test_adjoint_uneven_grids_match_oracle():
