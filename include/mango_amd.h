@@ -409,6 +409,17 @@ MangoStatus mango_price_table_validate(const MangoPriceTable* t, const MangoPric
  * extrapolates by clamping to the axes). Replaces mango_c_api.h:174-175. */
 double mango_price_table_price(const MangoPriceTable* t, const MangoPricingParams* p);
 double mango_price_table_vega(const MangoPriceTable* t, const MangoPricingParams* p);
+/* EEPLayer Greeks over the table (eep_layer.hpp:33-77): the params are
+ * validated against the table first (MANGO_ERR_VALIDATION outside it).
+ * Replace mango_c_api.h:176-183. */
+MangoStatus mango_price_table_delta(const MangoPriceTable* t, const MangoPricingParams* p, double* out,
+                                    MangoError* err);
+MangoStatus mango_price_table_gamma(const MangoPriceTable* t, const MangoPricingParams* p, double* out,
+                                    MangoError* err);
+MangoStatus mango_price_table_theta(const MangoPriceTable* t, const MangoPricingParams* p, double* out,
+                                    MangoError* err);
+MangoStatus mango_price_table_rho(const MangoPriceTable* t, const MangoPricingParams* p, double* out,
+                                  MangoError* err);
 MangoOptionType mango_price_table_option_type(const MangoPriceTable* t);
 double mango_price_table_dividend_yield(const MangoPriceTable* t);
 /* AnyPriceTable::make_iv_solver; cfg nullable (defaults). Replaces
@@ -469,6 +480,11 @@ MangoStatus mango_price_table_price_batch(const MangoPriceTable* t, const MangoP
  * sigma, r), outputs d_price and d_vega (nullable). */
 MangoStatus mango_price_table_price_device(const MangoPriceTable* t, const double* d_points, uint64_t n,
                                            double* d_price, double* d_vega, void* stream, MangoError* err);
+
+/* Delta, gamma, theta, rho at n points (device buffers, d_out [n][4]),
+ * asynchronous on `stream`; no domain validation (clamped like price). */
+MangoStatus mango_price_table_greeks_device(const MangoPriceTable* t, const double* d_points, uint64_t n,
+                                            double* d_out, void* stream, MangoError* err);
 
 /* Interpolated IV over queries resident in device memory (56-byte records)
  * into 24-byte results, asynchronous on `stream`. status as the slots of

@@ -110,4 +110,48 @@ __host__ __device__ inline void cubic_basis(const double* __restrict__ t, int nk
     }
 }
 
+// Second derivative of the cubic basis: degree-0/1 basis, first derivative
+// of the quadratic basis, then the cubic's second derivative.
+__host__ __device__ inline void cubic_basis_d2(const double* __restrict__ t, int nk, int i, double x, double N[4]) {
+    double N0[4], N1[4], dN2[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int idx = i - k;
+        N0[k] = (idx >= 0 && idx + 1 < nk && t[idx] <= x && x < t[idx + 1]) ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int idx = i - k;
+        N1[k] = 0.0;
+        if (idx >= 0 && idx + 2 < nk) {
+            const double ld = t[idx + 1] - t[idx], rd = t[idx + 2] - t[idx + 1];
+            const double l = ld > 0.0 ? (x - t[idx]) / ld * N0[k] : 0.0;
+            const double r = (rd > 0.0 && k > 0) ? (t[idx + 2] - x) / rd * N0[k > 0 ? k - 1 : 0] : 0.0;
+            N1[k] = l + r;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int idx = i - k;
+        dN2[k] = 0.0;
+        if (idx >= 0 && idx + 3 < nk) {
+            const double ld = t[idx + 2] - t[idx], rd = t[idx + 3] - t[idx + 1];
+            const double l = ld > 0.0 ? (2.0 / ld) * N1[k] : 0.0;
+            const double r = (rd > 0.0 && k > 0) ? (2.0 / rd) * N1[k > 0 ? k - 1 : 0] : 0.0;
+            dN2[k] = l - r;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int idx = i - k;
+        N[k] = 0.0;
+        if (idx >= 0 && idx + 4 < nk) {
+            const double ld = t[idx + 3] - t[idx], rd = t[idx + 4] - t[idx + 1];
+            const double l = ld > 0.0 ? (3.0 / ld) * dN2[k] : 0.0;
+            const double r = (rd > 0.0 && k > 0) ? (3.0 / rd) * dN2[k > 0 ? k - 1 : 0] : 0.0;
+            N[k] = l - r;
+        }
+    }
+}
+
 }  // namespace mango_amd

@@ -125,6 +125,32 @@ __device__ __forceinline__ double european_vega(double S, double K, double tau, 
     return S * exp(-q * tau) * sqrt(tau) * norm_pdf(d1);
 }
 
+// EuropeanOptionResult delta, gamma, theta, rho
+__device__ __forceinline__ double bs_d1(double S, double K, double tau, double sigma, double r, double q) {
+    const double sst = sigma * sqrt(tau);
+    return (log(S / K) + (r - q + 0.5 * sigma * sigma) * tau) / sst;
+}
+
+__device__ __forceinline__ void european_greeks(double S, double K, double tau, double sigma, double r, double q,
+                                                bool put, double out[4]) {
+    const double eq = exp(-q * tau);
+    if (tau <= 0.0 || sigma <= 0.0) {
+        out[0] = put ? (S < K ? -eq : 0.0) : (S > K ? eq : 0.0);
+        out[1] = out[2] = out[3] = 0.0;
+        return;
+    }
+    const double d1 = bs_d1(S, K, tau, sigma, r, q);
+    const double d2 = d1 - sigma * sqrt(tau);
+    const double sqrt_tau = sqrt(tau);
+    const double er = exp(-r * tau);
+    out[0] = put ? -eq * norm_cdf(-d1) : eq * norm_cdf(d1);
+    out[1] = eq * norm_pdf(d1) / (S * sigma * sqrt(tau));
+    const double common = -S * eq * norm_pdf(d1) * sigma / (2.0 * sqrt_tau);
+    out[2] = put ? common + r * K * er * norm_cdf(-d2) - q * S * eq * norm_cdf(-d1)
+                 : common - r * K * er * norm_cdf(d2) + q * S * eq * norm_cdf(d1);
+    out[3] = put ? -K * tau * er * norm_cdf(-d2) : K * tau * er * norm_cdf(d2);
+}
+
 // softplus with sharpness 100 minus its value at 0, floored at 0; the
 // identity once 100 x exceeds 500
 __device__ __forceinline__ double eep_floor(double x) {
@@ -221,15 +247,17 @@ struct AxisWeights {
     double w[4];
 };
 
-// clamp_bspline_query (bspline_nd.hpp:40-62) then span and basis of one axis
-__device__ __forceinline__ AxisWeights axis_weights(const DeviceTable& T, int a, double x, bool deriv) {
+// clamp_bspline_query (bspline_nd.hpp:40-62) then span and basis of one
+// axis; order 0 value, 1 first derivative, 2 second derivative.
+__device__ __forceinline__ AxisWeights axis_weights(const DeviceTable& T, int a, double x, int order) {
     const double* g = T.grid[a];
     const int n = T.n[a];
     if (x <= g[0]) x = g[0];
     else if (x >= g[n - 1]) x = nextafter(g[n - 1], -INFINITY);
     AxisWeights aw;
     aw.span = find_span(T.knots[a], n, x);
-    cubic_basis(T.knots[a], n + 4, aw.span, x, aw.w, deriv);
+    if (order == 2) cubic_basis_d2(T.knots[a], n + 4, aw.span, x, aw.w);
+    else cubic_basis(T.knots[a], n + 4, aw.span, x, aw.w, order == 1);
     return aw;
 }
 
@@ -278,29 +306,63 @@ struct TablePoint {
 
 __device__ __forceinline__ TablePoint table_point(const DeviceTable& T, double S, double K, double tau, double r) {
     TablePoint p;
-    p.m = axis_weights(T, 0, log(S / K), false);
-    p.tau = axis_weights(T, 1, tau, false);
-    p.r = axis_weights(T, 3, r, false);
+    p.m = axis_weights(T, 0, log(S / K), 0);
+    p.tau = axis_weights(T, 1, tau, 0);
+    p.r = axis_weights(T, 3, r, 0);
     p.S = S; p.K = K; p.tau_v = tau; p.r_v = r;
     return p;
 }
 
 __device__ __forceinline__ double point_price(const DeviceTable& T, const TablePoint& p, double sigma) {
-    const AxisWeights s = axis_weights(T, 2, sigma, false);
+    const AxisWeights s = axis_weights(T, 2, sigma, 0);
     const double raw = tensor_sum(T, p.m, p.tau, s, p.r);
     return (0.0 < raw ? raw : 0.0) * p.K / T.K_ref +
            european_price(p.S, p.K, p.tau_v, sigma, p.r_v, T.dividend_yield, T.option_type == 1);
 }
 
 __device__ __forceinline__ double point_vega(const DeviceTable& T, const TablePoint& p, double sigma) {
-    const AxisWeights s = axis_weights(T, 2, sigma, false);
+    const AxisWeights s = axis_weights(T, 2, sigma, 0);
     const double raw = tensor_sum(T, p.m, p.tau, s, p.r);
     double leaf = 0.0;
     if (!(raw <= 0.0)) {
-        const AxisWeights ds = axis_weights(T, 2, sigma, true);
+        const AxisWeights ds = axis_weights(T, 2, sigma, 1);
         leaf = tensor_sum(T, p.m, p.tau, ds, p.r) * p.K / T.K_ref;
     }
     return leaf + european_vega(p.S, p.K, p.tau_v, sigma, p.r_v, T.dividend_yield);
+}
+
+// EEPLayer::greek / gamma (eep_layer.hpp:33-77) over TransformLeaf
+// (transform_leaf.hpp:48-90) with the StandardTransform4D weights: delta =
+// f_x / S, theta = -f_tau, rho = f_r, gamma = (f_xx - f_x) / S^2, each times
+// K / K_ref, plus the European Greek; the European Greek alone where the EEP
+// reads <= 0.
+__device__ __forceinline__ void point_greeks(const DeviceTable& T, double S, double K, double tau, double sigma,
+                                             double r, double out[4]) {
+    const double x = log(S / K);
+    const AxisWeights m0 = axis_weights(T, 0, x, 0), t0 = axis_weights(T, 1, tau, 0);
+    const AxisWeights s0 = axis_weights(T, 2, sigma, 0), r0 = axis_weights(T, 3, r, 0);
+    const double raw = tensor_sum(T, m0, t0, s0, r0);
+    european_greeks(S, K, tau, sigma, r, T.dividend_yield, T.option_type == 1, out);
+    if (raw <= 0.0) return;
+    const double fx = tensor_sum(T, axis_weights(T, 0, x, 1), t0, s0, r0);
+    const double fxx = tensor_sum(T, axis_weights(T, 0, x, 2), t0, s0, r0);
+    const double ft = tensor_sum(T, m0, axis_weights(T, 1, tau, 1), s0, r0);
+    const double fr = tensor_sum(T, m0, t0, s0, axis_weights(T, 3, r, 1));
+    out[0] = (0.0 + (1.0 / S) * fx) * K / T.K_ref + out[0];
+    out[1] = (fxx - fx) / (S * S) * K / T.K_ref + out[1];
+    out[2] = (0.0 + -1.0 * ft) * K / T.K_ref + out[2];
+    out[3] = (0.0 + 1.0 * fr) * K / T.K_ref + out[3];
+}
+
+__global__ __launch_bounds__(kBlock) void table_greeks_kernel(DeviceTable T, const double* __restrict__ rows,
+                                                              int64_t n, double* __restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* q = rows + 5 * i;
+    double g[4];
+    point_greeks(T, q[0], q[1], q[2], q[3], q[4], g);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[4 * i + k] = g[k];
 }
 
 __global__ __launch_bounds__(kBlock) void table_price_kernel(DeviceTable T, const double* __restrict__ rows, int64_t n,
@@ -481,6 +543,13 @@ int launch_table_price(const DeviceTable& t, const double* d_rows, int64_t n, do
     if (n <= 0) return 0;
     hipLaunchKernelGGL(table_price_kernel, dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                        stream, t, d_rows, n, d_price, d_vega);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_table_greeks(const DeviceTable& t, const double* d_rows, int64_t n, double* d_out, hipStream_t stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(table_greeks_kernel, dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       stream, t, d_rows, n, d_out);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

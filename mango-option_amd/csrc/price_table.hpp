@@ -97,6 +97,10 @@ int launch_fit_axis(const double* d_src, double* d_dst, int64_t total, int na, i
 int launch_table_price(const DeviceTable& t, const double* d_rows, int64_t n, double* d_price, double* d_vega,
                        hipStream_t stream);
 
+// Table delta, gamma, theta, rho at points rows[i] = (S, K, tau, sigma, r):
+// out[4 i + k].
+int launch_table_greeks(const DeviceTable& t, const double* d_rows, int64_t n, double* d_out, hipStream_t stream);
+
 // Interpolated IV for n queries.
 int launch_interp_iv(const DeviceTable& t, const IvConfig& cfg, const IvQueryRec* d_q, int64_t n,
                      IvResultRec* d_out, hipStream_t stream);

@@ -698,6 +698,29 @@ double price_one(const MangoPriceTable* t, const MangoPricingParams* p, bool wan
     return want_vega ? vg[0] : pr[0];
 }
 
+// greek_call (mango_c_api.cpp:272-300): validate the params against the
+// table, then evaluate; k = 0 delta, 1 gamma, 2 theta, 3 rho.
+MangoStatus greek_one(const MangoPriceTable* t, const MangoPricingParams* p, double* out, MangoError* err, int k) {
+    if (!t || !p || !out) return fail(err, MANGO_ERR_VALIDATION, "null table, params, or out");
+    const MangoStatus v = mango_price_table_validate(t, p, err);
+    if (v != MANGO_OK) return v;
+    double row[5];
+    if (!price_row(p, row)) return fail(err, MANGO_ERR_VALIDATION, "invalid pricing parameters");
+    if (!device_ready(err)) return MANGO_ERR_SOLVER;
+    std::lock_guard<std::mutex> lk(g_query_mu);
+    DeviceArena a;
+    const size_t orow = a.reserve<double>(5), og = a.reserve<double>(4);
+    if (!a.allocate()) return fail(err, MANGO_ERR_SOLVER, "hipMalloc failed (table Greeks)");
+    double g[4];
+    if (hipMemcpy(a.at<double>(orow), row, sizeof row, hipMemcpyHostToDevice) != hipSuccess ||
+        launch_table_greeks(t->t->view, a.at<double>(orow), 1, a.at<double>(og), nullptr) != 0 ||
+        hipMemcpy(g, a.at<double>(og), sizeof g, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(err, MANGO_ERR_SOLVER, "table Greeks kernel failed");
+    if (!std::isfinite(g[k])) return fail(err, MANGO_ERR_SOLVER, "greek numerical computation failed");
+    *out = g[k];
+    return MANGO_OK;
+}
+
 const char* iv_status_message(int32_t st) {
     switch (st) {
         case MANGO_ERR_VALIDATION: return "interpolated IV: invalid query or outside the table";
@@ -839,6 +862,24 @@ MangoStatus mango_price_table_price_batch(const MangoPriceTable* t, const MangoP
         out_price[i] = ok[i] ? pr[i] : nan;
         if (out_vega) out_vega[i] = ok[i] ? vg[i] : nan;
     }
+    return MANGO_OK;
+}
+
+MangoStatus mango_price_table_delta(const MangoPriceTable* t, const MangoPricingParams* p, double* out,
+                                    MangoError* err) { return greek_one(t, p, out, err, 0); }
+MangoStatus mango_price_table_gamma(const MangoPriceTable* t, const MangoPricingParams* p, double* out,
+                                    MangoError* err) { return greek_one(t, p, out, err, 1); }
+MangoStatus mango_price_table_theta(const MangoPriceTable* t, const MangoPricingParams* p, double* out,
+                                    MangoError* err) { return greek_one(t, p, out, err, 2); }
+MangoStatus mango_price_table_rho(const MangoPriceTable* t, const MangoPricingParams* p, double* out,
+                                  MangoError* err) { return greek_one(t, p, out, err, 3); }
+
+MangoStatus mango_price_table_greeks_device(const MangoPriceTable* t, const double* d_points, uint64_t n,
+                                            double* d_out, void* stream, MangoError* err) {
+    if (!t || (n && (!d_points || !d_out))) return fail(err, MANGO_ERR_VALIDATION, "null table, points or output");
+    if (!device_ready(err)) return MANGO_ERR_SOLVER;
+    if (launch_table_greeks(t->t->view, d_points, static_cast<int64_t>(n), d_out, static_cast<hipStream_t>(stream)) != 0)
+        return fail(err, MANGO_ERR_SOLVER, "table Greeks launch failed");
     return MANGO_OK;
 }
 

@@ -287,6 +287,12 @@ SIGNATURES = {
     "mango_price_table_validate": (C.c_int32, [C.c_void_p, _P(MangoPricingParams), _P(MangoError)]),
     "mango_price_table_price": (C.c_double, [C.c_void_p, _P(MangoPricingParams)]),
     "mango_price_table_vega": (C.c_double, [C.c_void_p, _P(MangoPricingParams)]),
+    "mango_price_table_delta": (C.c_int32, [C.c_void_p, _P(MangoPricingParams), _P(C.c_double), _P(MangoError)]),
+    "mango_price_table_gamma": (C.c_int32, [C.c_void_p, _P(MangoPricingParams), _P(C.c_double), _P(MangoError)]),
+    "mango_price_table_theta": (C.c_int32, [C.c_void_p, _P(MangoPricingParams), _P(C.c_double), _P(MangoError)]),
+    "mango_price_table_rho": (C.c_int32, [C.c_void_p, _P(MangoPricingParams), _P(C.c_double), _P(MangoError)]),
+    "mango_price_table_greeks_device": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                                    _P(MangoError)]),
     "mango_price_table_option_type": (C.c_int32, [C.c_void_p]),
     "mango_price_table_dividend_yield": (C.c_double, [C.c_void_p]),
     "mango_price_table_make_iv_solver": (C.c_int32, [C.c_void_p, _P(MangoInterpSolverConfig), _P(C.c_void_p),

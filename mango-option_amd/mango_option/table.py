@@ -265,6 +265,41 @@ class PriceTable:
         self._validate(params)
         return float(self._lib.mango_price_table_vega(self._h, C.byref(_c_pricing(params))))
 
+    def _greek(self, name, params) -> float:
+        out = C.c_double()
+        err = MangoError()
+        rc = getattr(self._lib, f"mango_price_table_{name}")(self._h, C.byref(_c_pricing(params)), C.byref(out),
+                                                             C.byref(err))
+        _raise(rc, err, name)
+        return out.value
+
+    def delta(self, params) -> float:
+        return self._greek("delta", params)
+
+    def gamma(self, params) -> float:
+        return self._greek("gamma", params)
+
+    def theta(self, params) -> float:
+        return self._greek("theta", params)
+
+    def rho(self, params) -> float:
+        return self._greek("rho", params)
+
+    def greeks_points(self, rows):
+        """rows [n, 5] = (S, K, tau, sigma, r) -> [n, 4] delta, gamma, theta,
+        rho (numpy), evaluated on the device without domain validation."""
+        import torch
+        rows = np.ascontiguousarray(rows, dtype=np.float64).reshape(-1, 5)
+        d_rows = torch.from_numpy(rows).cuda()
+        d_out = torch.empty((rows.shape[0], 4), dtype=torch.float64, device=d_rows.device)
+        err = MangoError()
+        stream = C.c_void_p(torch.cuda.current_stream(d_rows.device).cuda_stream)
+        rc = self._lib.mango_price_table_greeks_device(self._h, C.c_void_p(d_rows.data_ptr()), rows.shape[0],
+                                                       C.c_void_p(d_out.data_ptr()), stream, C.byref(err))
+        _raise(rc, err, "mango_price_table_greeks_device")
+        torch.cuda.synchronize()
+        return d_out.cpu().numpy()
+
     def make_iv_solver(self, solver_config: InterpolatedIVSolverConfig | None = None) -> InterpolatedIVSolver:
         out = C.c_void_p()
         err = MangoError()

@@ -123,6 +123,48 @@ inline void cubic_basis(const std::vector<double>& t, int i, double x, double N[
     }
 }
 
+// Second derivative of the cubic basis (cubic_basis_second_derivative_nonuniform,
+// bspline_basis.hpp:343-409): degree-0/1 basis, first derivative of the
+// quadratic basis, then the cubic's second derivative.
+inline void cubic_basis_d2(const std::vector<double>& t, int i, double x, double N[4]) {
+    const int n = static_cast<int>(t.size());
+    double N0[4] = {0, 0, 0, 0}, N1[4] = {0, 0, 0, 0}, dN2[4] = {0, 0, 0, 0};
+    auto T = [&](int j) { return t[static_cast<size_t>(j)]; };
+    for (int k = 0; k < 4; ++k) {
+        const int idx = i - k;
+        if (idx >= 0 && idx + 1 < n) N0[k] = (T(idx) <= x && x < T(idx + 1)) ? 1.0 : 0.0;
+    }
+    for (int k = 0; k < 4; ++k) {
+        const int idx = i - k;
+        if (idx >= 0 && idx + 2 < n) {
+            const double ld = T(idx + 1) - T(idx), rd = T(idx + 2) - T(idx + 1);
+            const double l = ld > 0.0 ? (x - T(idx)) / ld * N0[k] : 0.0;
+            const double r = (rd > 0.0 && k > 0) ? (T(idx + 2) - x) / rd * N0[k - 1] : 0.0;
+            N1[k] = l + r;
+        }
+    }
+    for (int k = 0; k < 4; ++k) {
+        const int idx = i - k;
+        if (idx >= 0 && idx + 3 < n) {
+            const double ld = T(idx + 2) - T(idx), rd = T(idx + 3) - T(idx + 1);
+            const double l = ld > 0.0 ? (2.0 / ld) * N1[k] : 0.0;
+            const double r = (rd > 0.0 && k > 0) ? (2.0 / rd) * N1[k - 1] : 0.0;
+            dN2[k] = l - r;
+        }
+    }
+    for (int k = 0; k < 4; ++k) {
+        const int idx = i - k;
+        if (idx >= 0 && idx + 4 < n) {
+            const double ld = T(idx + 3) - T(idx), rd = T(idx + 4) - T(idx + 1);
+            const double l = ld > 0.0 ? (3.0 / ld) * dN2[k] : 0.0;
+            const double r = (rd > 0.0 && k > 0) ? (3.0 / rd) * dN2[k - 1] : 0.0;
+            N[k] = l - r;
+        } else {
+            N[k] = 0.0;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 1-D collocation: B c = f, B[i][j] = N_j(x_i)
 // ---------------------------------------------------------------------------
@@ -222,7 +264,8 @@ struct BSpline4 {
         coeffs = std::move(c);
     }
     // clamp_bspline_query + tensor-product sum (bspline_nd.hpp:40-62, 133-160, 287-314)
-    double eval(const double q[4], int partial_axis = -1) const {
+    // order 1: first partial along partial_axis, 2: second partial
+    double eval(const double q[4], int partial_axis = -1, int order = 1) const {
         int span[4];
         double w[4][4];
         for (int a = 0; a < 4; ++a) {
@@ -231,7 +274,8 @@ struct BSpline4 {
             if (x <= g.front()) x = g.front();
             else if (x >= g.back()) x = std::nextafter(g.back(), -std::numeric_limits<double>::infinity());
             span[a] = find_span_cubic(knots[static_cast<size_t>(a)], x);
-            cubic_basis(knots[static_cast<size_t>(a)], span[a], x, w[a], a == partial_axis);
+            if (a == partial_axis && order == 2) cubic_basis_d2(knots[static_cast<size_t>(a)], span[a], x, w[a]);
+            else cubic_basis(knots[static_cast<size_t>(a)], span[a], x, w[a], a == partial_axis);
         }
         const size_t d1 = grids[1].size(), d2 = grids[2].size(), d3 = grids[3].size();
         const int dims[4] = {static_cast<int>(grids[0].size()), static_cast<int>(d1), static_cast<int>(d2), static_cast<int>(d3)};
@@ -288,6 +332,43 @@ inline double european_vega(double S, double K, double tau, double sigma, double
     return bs_vega(S, K, tau, sigma, r, q);
 }
 
+// EuropeanOptionResult delta, gamma, theta, rho (european_option.cpp:60-160)
+inline double bs_d1(double S, double K, double tau, double sigma, double r, double q) {
+    const double sst = sigma * std::sqrt(tau);
+    return (std::log(S / K) + (r - q + 0.5 * sigma * sigma) * tau) / sst;
+}
+inline double european_delta(double S, double K, double tau, double sigma, double r, double q, OptionType type) {
+    const double eq = std::exp(-q * tau);
+    if (tau <= 0.0 || sigma <= 0.0) {
+        if (type == OptionType::PUT) return (S < K) ? -eq : 0.0;
+        return (S > K) ? eq : 0.0;
+    }
+    const double d1 = bs_d1(S, K, tau, sigma, r, q);
+    return type == OptionType::PUT ? -eq * norm_cdf(-d1) : eq * norm_cdf(d1);
+}
+inline double european_gamma(double S, double K, double tau, double sigma, double r, double q) {
+    if (tau <= 0.0 || sigma <= 0.0) return 0.0;
+    const double d1 = bs_d1(S, K, tau, sigma, r, q);
+    return std::exp(-q * tau) * norm_pdf(d1) / (S * sigma * std::sqrt(tau));
+}
+inline double european_theta(double S, double K, double tau, double sigma, double r, double q, OptionType type) {
+    if (tau <= 0.0 || sigma <= 0.0) return 0.0;
+    const double d1 = bs_d1(S, K, tau, sigma, r, q);
+    const double d2 = d1 - sigma * std::sqrt(tau);
+    const double sqrt_tau = std::sqrt(tau);
+    const double eq = std::exp(-q * tau), er = std::exp(-r * tau);
+    const double common = -S * eq * norm_pdf(d1) * sigma / (2.0 * sqrt_tau);
+    if (type == OptionType::PUT) return common + r * K * er * norm_cdf(-d2) - q * S * eq * norm_cdf(-d1);
+    return common - r * K * er * norm_cdf(d2) + q * S * eq * norm_cdf(d1);
+}
+inline double european_rho(double S, double K, double tau, double sigma, double r, double q, OptionType type) {
+    if (tau <= 0.0 || sigma <= 0.0) return 0.0;
+    const double d1 = bs_d1(S, K, tau, sigma, r, q);
+    const double d2 = d1 - sigma * std::sqrt(tau);
+    const double er = std::exp(-r * tau);
+    return type == OptionType::PUT ? -K * tau * er * norm_cdf(-d2) : K * tau * er * norm_cdf(d2);
+}
+
 // ---------------------------------------------------------------------------
 // Price table: EEP B-spline over (ln(S/K), tau, sigma, r) at K_ref
 // ---------------------------------------------------------------------------
@@ -306,6 +387,32 @@ struct PriceTable {
         const double raw = spline.eval(c);
         const double leaf = raw <= 0.0 ? 0.0 : spline.eval(c, 2) * K / K_ref;
         return leaf + european_vega(S, K, tau, sigma, r, q);
+    }
+    // EEPLayer::greek / gamma over TransformLeaf with StandardTransform4D
+    // weights (eep_layer.hpp:33-77, transform_leaf.hpp:48-90,
+    // transforms/standard_4d.hpp:20-31): delta = f_x / S, theta = -f_tau,
+    // rho = f_r, gamma = (f_xx - f_x) / S^2, each times K / K_ref, plus the
+    // European Greek; the European Greek alone where the EEP reads <= 0.
+    void greeks(double S, double K, double tau, double sigma, double r, double out[4]) const {
+        const double c[4] = {std::log(S / K), tau, sigma, r};
+        const double raw = spline.eval(c);
+        out[0] = european_delta(S, K, tau, sigma, r, q, type);
+        out[1] = european_gamma(S, K, tau, sigma, r, q);
+        out[2] = european_theta(S, K, tau, sigma, r, q, type);
+        out[3] = european_rho(S, K, tau, sigma, r, q, type);
+        if (raw <= 0.0) return;
+        const double fx = spline.eval(c, 0);
+        const double fxx = spline.eval(c, 0, 2);
+        double d = 0.0;
+        d += (1.0 / S) * fx;
+        out[0] = d * K / K_ref + out[0];
+        out[1] = (fxx - fx) / (S * S) * K / K_ref + out[1];
+        double th = 0.0;
+        th += -1.0 * spline.eval(c, 1);
+        out[2] = th * K / K_ref + out[2];
+        double rh = 0.0;
+        rh += 1.0 * spline.eval(c, 3);
+        out[3] = rh * K / K_ref + out[3];
     }
 };
 

@@ -384,6 +384,30 @@ void oracle_table_price(const double* a0, uint64_t n0, const double* a1, uint64_
     }
 }
 
+// Table delta, gamma, theta, rho at m points (rows S, K, tau, sigma, r).
+void oracle_table_greeks(const double* a0, uint64_t n0, const double* a1, uint64_t n1, const double* a2, uint64_t n2,
+                         const double* a3, uint64_t n3, const double* coeffs, double K_ref, double q, int32_t type,
+                         const double* rows, uint64_t m, double* out) {
+    PriceTable t;
+    t.spline.set(axes4(a0, n0, a1, n1, a2, n2, a3, n3), std::vector<double>(coeffs, coeffs + n0 * n1 * n2 * n3));
+    t.K_ref = K_ref;
+    t.q = q;
+    t.type = type == MANGO_PUT ? OptionType::PUT : OptionType::CALL;
+    for (uint64_t i = 0; i < m; ++i) {
+        const double* r = rows + 5 * i;
+        t.greeks(r[0], r[1], r[2], r[3], r[4], out + 4 * i);
+    }
+}
+
+// B-spline second partial along `axis` at m points q[m][4].
+void oracle_bspline_eval4_d2(const double* a0, uint64_t n0, const double* a1, uint64_t n1, const double* a2,
+                             uint64_t n2, const double* a3, uint64_t n3, const double* coeffs, const double* q,
+                             uint64_t m, int32_t axis, double* out) {
+    BSpline4 b;
+    b.set(axes4(a0, n0, a1, n1, a2, n2, a3, n3), std::vector<double>(coeffs, coeffs + n0 * n1 * n2 * n3));
+    for (uint64_t i = 0; i < m; ++i) out[i] = b.eval(q + 4 * i, axis, 2);
+}
+
 int32_t oracle_thomas(const double* lo, const double* dg, const double* up, const double* rhs,
                       double* sol, uint64_t n) {
     std::vector<double> ws(2 * n + 2);

@@ -66,6 +66,10 @@ def load() -> C.CDLL:
         "oracle_iv_batch": (None, [_dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_double, C.c_double, C.c_int32, _dp, P(C.c_int32), C.c_uint64,
                                    _dp, _dp, P(C.c_uint64), _dp, P(C.c_int32), C.c_int32]),
         "oracle_table_price": (None, [_dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_double, C.c_double, C.c_int32, _dp, C.c_uint64, _dp, _dp]),
+        "oracle_table_greeks": (None, [_dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp,
+                                       C.c_double, C.c_double, C.c_int32, _dp, C.c_uint64, _dp]),
+        "oracle_bspline_eval4_d2": (None, [_dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp, C.c_uint64, _dp,
+                                           _dp, C.c_uint64, C.c_int32, _dp]),
         "oracle_thomas": (C.c_int32, [_dp, _dp, _dp, _dp, _dp, C.c_uint64]),
         "oracle_thomas_projected": (C.c_int32, [_dp, _dp, _dp, _dp, _dp, _dp, C.c_uint64]),
         "oracle_spline_eval": (C.c_int32, [_dp, _dp, C.c_uint64, _dp, C.c_uint64, _dp, _dp]),
@@ -249,6 +253,14 @@ class Table:
                                   _p(rows), m, _p(pr), _p(vg))
         return pr, vg
 
+    def greeks(self, rows):
+        """rows [m, 5] = S, K, tau, sigma, r -> [m, 4] = delta, gamma, theta, rho."""
+        rows = _a(rows).reshape(-1, 5)
+        out = np.empty((rows.shape[0], 4))
+        load().oracle_table_greeks(*_axes_args(self.axes), _p(self.coeffs), self.K_ref, self.q,
+                                   1 if self.put else 0, _p(rows), rows.shape[0], _p(out))
+        return out
+
     def iv(self, rows, put=None, cfg=None, n_threads=1):
         """rows [n, 6] = S, K, tau, r, q, market -> (iv, iterations, final_error,
         status); cfg = (max_iter, tolerance, sigma_min, sigma_max, vega_threshold)."""
@@ -313,3 +325,12 @@ def eep_tensor(axes, K_ref, q, put, prices):
     out = np.empty_like(v)
     load().oracle_eep_tensor(*_axes_args(axes), float(K_ref), float(q), 1 if put else 0, _p(v), _p(out))
     return out.reshape([len(a) for a in axes])
+
+
+def bspline_eval4_d2(axes, coeffs, points, axis):
+    axes = [_a(a) for a in axes]
+    c = _a(coeffs)
+    pts = _a(points).reshape(-1, 4)
+    out = np.empty(pts.shape[0])
+    load().oracle_bspline_eval4_d2(*_axes_args(axes), _p(c), _p(pts), pts.shape[0], int(axis), _p(out))
+    return out

@@ -223,3 +223,23 @@ def test_factory_table_accuracy_grid():
         M.make_price_table(M.IVSolverFactoryConfig(grid=M.IVGrid(moneyness=[0.9, 1.0, 1.1], vol=[0.2] * 4,
                                                                  rate=[0.0] * 4),
                                                    backend=M.BSplineBackend(maturity_grid=[0.5] * 4)))
+
+
+def test_c4_table_greeks(c4):
+    t, ref = c4
+    axes, coeffs, _ = t.export()
+    rows = _rows(2048, 33)
+    g = t.greeks_points(rows)
+    same = O.Table(axes, coeffs, W.C4_K_REF, W.C4_DIVIDEND, True)
+    want = same.greeks(rows)
+    assert np.all(np.abs(g - want) <= 1e-9 * np.maximum(1.0, np.abs(want)))
+    # the reference-named scalar entries validate against the table first
+    p = M.PricingParams(spot=95.0, strike=100.0, maturity=0.7, rate=0.04, dividend_yield=W.C4_DIVIDEND,
+                        option_type=M.OptionType.PUT, volatility=0.3)
+    w = same.greeks(np.array([[95.0, 100.0, 0.7, 0.3, 0.04]]))[0]
+    for k, name in enumerate(("delta", "gamma", "theta", "rho")):
+        assert abs(getattr(t, name)(p) - w[k]) <= 1e-9 * max(1.0, abs(w[k]))
+    out = M.PricingParams(spot=95.0, strike=100.0, maturity=3.0, rate=0.04, dividend_yield=W.C4_DIVIDEND,
+                          option_type=M.OptionType.PUT, volatility=0.3)
+    with pytest.raises(ValueError):
+        t.delta(out)

@@ -190,3 +190,75 @@ def test_c4_iv_statuses(c4_table):
     for name, (row, put, want) in cases.items():
         _, _, _, st = c4_table.iv(np.array([row]), put=put)
         assert st[0] == want, name
+
+
+def test_bspline_second_partial():
+    rng = np.random.default_rng(15)
+    ax = _grids(rng, (7, 6, 5, 5))
+    ca = [rng.normal(size=4) for _ in range(4)]
+    f = lambda a, x: ca[a][0] + x * (ca[a][1] + x * (ca[a][2] + x * ca[a][3]))
+    d2f = lambda a, x: 2 * ca[a][2] + 6 * x * ca[a][3]
+    X = np.meshgrid(*ax, indexing="ij")
+    coef, _ = O.bspline_fit4(ax, f(0, X[0]) * f(1, X[1]) * f(2, X[2]) * f(3, X[3]))
+    pts = np.stack([rng.uniform(-0.95, 0.95, 200) for _ in range(4)], 1)
+    for a in range(4):
+        want = np.prod([d2f(b, pts[:, b]) if b == a else f(b, pts[:, b]) for b in range(4)], axis=0)
+        np.testing.assert_allclose(O.bspline_eval4_d2(ax, coef, pts, a), want, rtol=0, atol=1e-9)
+    # arbitrary coefficients: central difference of the first partial
+    coef = rng.normal(size=[len(a) for a in ax])
+    h = 1e-6
+    e = np.zeros(4)
+    e[0] = h
+    fd = (O.bspline_eval4(ax, coef, pts + e, 0) - O.bspline_eval4(ax, coef, pts - e, 0)) / (2 * h)
+    np.testing.assert_allclose(O.bspline_eval4_d2(ax, coef, pts, 0), fd, rtol=1e-5, atol=1e-5)
+
+
+def test_table_greeks_match_price_differences(c4_table):
+    """delta, gamma, theta, rho of the table against central differences of
+    the table price (away from the EEP floor's kink, where both are smooth)."""
+    rng = np.random.default_rng(16)
+    n = 60
+    S = 100.0 * np.exp(rng.uniform(-0.3, 0.25, n))
+    K = np.full(n, 100.0)
+    tau = rng.uniform(0.3, 1.6, n)
+    sig = rng.uniform(0.25, 0.6, n)
+    r = rng.uniform(0.01, 0.09, n)
+    rows = np.stack([S, K, tau, sig, r], 1)
+    g = c4_table.greeks(rows)
+
+    def p(dS=0.0, dt=0.0, dr=0.0):
+        return c4_table.price(np.stack([S + dS, K, tau + dt, sig, r + dr], 1))[0]
+
+    hS, ht, hr = 1e-3 * S, 1e-5, 1e-6
+    delta = (p(dS=hS) - p(dS=-hS)) / (2 * hS)
+    gamma = (p(dS=hS) - 2 * p() + p(dS=-hS)) / hS ** 2
+    theta = -(p(dt=ht) - p(dt=-ht)) / (2 * ht)
+    rho = (p(dr=hr) - p(dr=-hr)) / (2 * hr)
+    np.testing.assert_allclose(g[:, 0], delta, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(g[:, 1], gamma, rtol=1e-3, atol=1e-5)
+    np.testing.assert_allclose(g[:, 2], theta, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(g[:, 3], rho, rtol=1e-4, atol=1e-4)
+
+
+def test_european_greek_identities(c4_table):
+    """Where the EEP reads zero the table Greeks are the Black-Scholes Greeks:
+    check put-call relations of the closed forms through a call and a put
+    table of zero coefficients."""
+    ax = W.c4_axes()
+    zero = np.zeros([len(a) for a in ax])
+    put = O.Table(ax, zero, 100.0, 0.02, True)
+    call = O.Table(ax, zero, 100.0, 0.02, False)
+    rows = np.array([[95.0, 100.0, 0.8, 0.3, 0.04], [120.0, 100.0, 1.5, 0.2, 0.01], [70.0, 100.0, 0.1, 0.5, 0.08]])
+    gp, gc = put.greeks(rows), call.greeks(rows)
+    S, K, tau, r, q = rows[:, 0], rows[:, 1], rows[:, 2], rows[:, 4], 0.02
+    np.testing.assert_allclose(gc[:, 0] - gp[:, 0], np.exp(-q * tau), rtol=1e-12)
+    np.testing.assert_allclose(gc[:, 1], gp[:, 1], rtol=1e-12)
+    np.testing.assert_allclose(gc[:, 3] - gp[:, 3], K * tau * np.exp(-r * tau), rtol=1e-12)
+    # theta of the parity C - P = S e^{-q tau} - K e^{-r tau}: d/dt = q S e^{-q tau} - r K e^{-r tau}
+    np.testing.assert_allclose(gc[:, 2] - gp[:, 2], q * S * np.exp(-q * tau) - r * K * np.exp(-r * tau),
+                               rtol=1e-10, atol=1e-12)
+    # gamma: central difference of the closed-form delta
+    h = 1e-4 * S
+    up = put.greeks(np.stack([S + h, K, tau, rows[:, 3], r], 1))[:, 0]
+    dn = put.greeks(np.stack([S - h, K, tau, rows[:, 3], r], 1))[:, 0]
+    np.testing.assert_allclose(gp[:, 1], (up - dn) / (2 * h), rtol=1e-6)
