@@ -25,6 +25,8 @@
 // solves' worth of scans per step).
 #include "american_device.hpp"
 
+#include <cstdlib>
+
 namespace mango_amd {
 namespace {
 
@@ -100,6 +102,7 @@ __device__ __forceinline__ void adjoint_one(const PdeJob* __restrict__ jobs, int
     const int base = P.base;
     const int64_t N = P.steps(job);
     const double dt = job.maturity / static_cast<double>(N);
+    P.set_dt(dt);
     int64_t S = static_cast<int64_t>(ceil(sqrt(static_cast<double>(N))));
     if (S < 1) S = 1;
     const int64_t n_ck = (N + S - 1) / S;
@@ -159,9 +162,15 @@ __device__ __forceinline__ void adjoint_one(const PdeJob* __restrict__ jobs, int
 #pragma unroll
     for (int k = 0; k < M; ++k) { gl[k] = gd[k] = gu[k] = 0.0; unext[k] = u[k]; }
     double rbc = 0.0;
+    DenAcc<M> a0, a1;  // stage-1 (and Rannacher) and stage-2 kinds
+    a0.clear();
+    a1.clear();
     for (int64_t seg = n_ck - 1; ok && seg >= 0; --seg) {
         const int64_t s0 = seg * S;
         const int64_t s1 = (s0 + S < N) ? s0 + S : N;
+        // the recompute may refactorize: settle the accumulated sources first
+        P.flush(c0, a0, gl, gd, gu);
+        P.flush(c1, a1, gl, gd, gu);
         double w[M];
         row_load<K, M>(ck + seg * NMAX, lane, w);
         for (int64_t s = s0; ok && s < s1; ++s) {
@@ -175,17 +184,23 @@ __device__ __forceinline__ void adjoint_one(const PdeJob* __restrict__ jobs, int
             row_load<K, M>(tr + (2 * (s - s0) + 1) * NMAX, lane, um);
             if (s == 0) {
                 const double half_dt = 0.5 * dt;
+                P.flush(c0, a0, gl, gd, gu);
                 c0.invalidate();
-                ok = P.stage_reverse(0.0 + dt, half_dt, um, um, c0, unext, ub, rbf, gl, gd, gu, rbc);
-                ok = ok && P.stage_reverse(0.0 + half_dt, half_dt, un, un, c0, um, rbf, unb, gl, gd, gu, rbc);
+                const double d1 = exp(-P.r * (0.0 + dt)), dh = exp(-P.r * (0.0 + half_dt));
+                ok = P.stage_reverse(0.0 + dt, d1, half_dt, um, um, c0, a0, unext, ub, rbf, gl, gd, gu, rbc);
+                ok = ok && P.stage_reverse(0.0 + half_dt, dh, half_dt, un, un, c0, a0, um, rbf, unb, gl, gd, gu,
+                                           rbc);
+                P.flush(c0, a0, gl, gd, gu);
                 c0.invalidate();
             } else {
                 const double t = static_cast<double>(s) * dt;
                 const double t_next = static_cast<double>(s + 1) * dt;
+                const double disc = P.put ? 0.0 : exp(-P.r * t);  // as the forward's advance()
                 const double w1 = trc.w1(dt);
 #pragma unroll
                 for (int k = 0; k < M; ++k) rhs[k] = fma(trc.alpha, um[k], trc.beta * un[k]);
-                ok = P.stage_reverse(t_next, trc.w2(dt), rhs, um, c1, unext, ub, rbf, gl, gd, gu, rbc);
+                ok = P.stage_reverse(t_next, disc * P.e_dt, trc.w2(dt), rhs, um, c1, a1, unext, ub, rbf, gl, gd, gu,
+                                     rbc);
                 double umb[M];
 #pragma unroll
                 for (int k = 0; k < M; ++k) {
@@ -193,13 +208,17 @@ __device__ __forceinline__ void adjoint_one(const PdeJob* __restrict__ jobs, int
                     unb[k] = trc.beta * rbf[k];
                 }
                 P.stage1_rhs(w1, un, rhs);
-                ok = ok && P.stage_reverse(t + trc.gamma * dt, w1, rhs, un, c0, um, umb, rbf, gl, gd, gu, rbc);
+                ok = ok && P.stage_reverse(t + trc.gamma * dt, disc * P.e_g, w1, rhs, un, c0, a0, um, umb, rbf, gl,
+                                           gd, gu, rbc);
                 P.stage1_rhs_reverse(w1, un, rbf, unb, gl, gd, gu);
             }
 #pragma unroll
             for (int k = 0; k < M; ++k) { ub[k] = unb[k]; unext[k] = un[k]; }
         }
     }
+
+    P.flush(c0, a0, gl, gd, gu);
+    P.flush(c1, a1, gl, gd, gu);
 
     // ---- 4. contract with the stencil weights ----
     double sa = 0.0, sb = 0.0, sc = 0.0;
@@ -279,10 +298,16 @@ int resident_blocks() {
 
 }  // namespace
 
-#define MANGO_ADJOINT_VARIANTS(X) X(32, 4) X(64, 4) X(64, 7)
+#define MANGO_ADJOINT_VARIANTS(X) X(32, 4) X(64, 2) X(64, 4) X(64, 7)
 
+// $MANGO_AMD_ADJOINT_LANES=64 selects one option per wavefront up to 128
+// points (64 x 2) instead of two (32 x 4).
 SolveVariant pick_adjoint_variant(int64_t n_space) {
-    if (n_space <= 128) return {32, 4};
+    static const bool wide = [] {
+        const char* v = std::getenv("MANGO_AMD_ADJOINT_LANES");
+        return v && std::atoi(v) == 64;
+    }();
+    if (n_space <= 128) return wide ? SolveVariant{64, 2} : SolveVariant{32, 4};
     if (n_space <= 256) return {64, 4};
     return {64, 7};
 }

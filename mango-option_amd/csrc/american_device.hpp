@@ -445,6 +445,24 @@ struct StageCache {
     }
 };
 
+// Sum over reverse stages of the denominator-adjoint sources of one stage
+// kind. While the kind's factorization (weight, lock pattern) is unchanged
+// the denominator-adjoint recurrence has fixed coefficients, so the sum of
+// its per-stage solutions is the solution for the summed sources: one scan
+// per flush instead of one per stage.
+template <int M>
+struct DenAcc {
+    double h[M];
+    double w;
+    bool active;
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (int k = 0; k < M; ++k) h[k] = 0.0;
+        w = 0.0;
+        active = false;
+    }
+};
+
 // TR-BDF2 constants (trbdf2_config.hpp:17-49; pde_solver.hpp:342-395).
 struct TrConstants {
     double gamma, alpha, beta;
@@ -471,6 +489,7 @@ struct OptionPde {
     bool put, valid, uniform;
     uint64_t gmask;
     double r, q, hss, drift, expected_dx, psi_max, dx_min, g_left, x_last;
+    double ex_last, e_dt, e_g;  // e^{x_{n-1}}; e^{-r dt}, e^{-r gamma dt} (calls, after set_dt)
     double psi[M], jl[M], jd[M], ju[M];
     uint32_t lockable;
 
@@ -561,6 +580,8 @@ struct OptionPde {
         const double x_first = valid ? xs[0] : 0.0;
         x_last = valid ? xs[n - 1] : 0.0;
         g_left = put ? smax(1.0 - exp(x_first), 0.0) : 0.0;
+        ex_last = put ? 0.0 : exp(x_last);
+        e_dt = e_g = 0.0;
         k_right = (n - 1 >= base && n - 1 < base + M) ? n - 1 - base : -1;
     }
 
@@ -580,9 +601,18 @@ struct OptionPde {
     __device__ __forceinline__ double payoff(double xv) const {
         return put ? smax(1.0 - exp(xv), 0.0) : smax(exp(xv) - 1.0, 0.0);
     }
-    __device__ __forceinline__ double g_right(double t) const { return put ? 0.0 : exp(x_last) - exp(-r * t); }
-    // d g_right / dr (the call's right boundary)
-    __device__ __forceinline__ double g_right_dr(double t) const { return put ? 0.0 : t * exp(-r * t); }
+    __device__ __forceinline__ double g_right(double t) const { return put ? 0.0 : ex_last - exp(-r * t); }
+    // the call boundary from the discount factor disc = e^{-r t}, and its d/dr
+    __device__ __forceinline__ double g_right_d(double disc) const { return put ? 0.0 : ex_last - disc; }
+    __device__ __forceinline__ double g_right_dr_d(double t, double disc) const { return put ? 0.0 : t * disc; }
+    // discount factors of a TR-BDF2 step of length dt: stage 1 at t + gamma dt
+    // and stage 2 at t + dt from e^{-r t} (one exp per step instead of two)
+    __device__ __forceinline__ void set_dt(double dt) {
+        if (put) return;
+        const double gamma = 2.0 - sqrt(2.0);
+        e_dt = exp(-r * dt);
+        e_g = exp(-r * (gamma * dt));
+    }
 
     // TimeDomain::from_n_steps; auto step count (grid_spec_types.cpp:90-99)
     __device__ __forceinline__ int64_t steps(const PdeJob& job) const {
@@ -611,10 +641,8 @@ struct OptionPde {
 
     // Stage matrix rows (Dirichlet rows, deep-ITM locks), c'/1/denom from the
     // cache (refreshed when the lock pattern changed) and d'.
-    __device__ __forceinline__ bool eliminate(double t, double w, const double (&rhs)[M], const double (&guess)[M],
-                                              StageCache<M>& c, double (&nlo)[M], double (&dp)[M], uint32_t& mask,
-                                              double& gr) const {
-        gr = g_right(t);
+    __device__ __forceinline__ bool eliminate(double gr, double w, const double (&rhs)[M], const double (&guess)[M],
+                                              StageCache<M>& c, double (&nlo)[M], double (&dp)[M], uint32_t& mask) const {
         double rb[M];
         mask = 0;
 #pragma unroll
@@ -697,11 +725,12 @@ struct OptionPde {
 
     // One projected implicit stage (solve_implicit_stage_projected,
     // pde_solver.hpp:525-729): u = projected solve, then the boundary values.
-    __device__ __forceinline__ bool stage(double t, double w, const double (&rhs)[M], const double (&guess)[M],
+    // gr: the right boundary value at the stage time.
+    __device__ __forceinline__ bool stage(double gr, double w, const double (&rhs)[M], const double (&guess)[M],
                                           StageCache<M>& c, double (&u)[M]) const {
-        double nlo[M], dp[M], gr;
+        double nlo[M], dp[M];
         uint32_t mask;
-        if (!eliminate(t, w, rhs, guess, c, nlo, dp, mask, gr)) return false;
+        if (!eliminate(gr, w, rhs, guess, c, nlo, dp, mask)) return false;
         double v;
         if (c.mono) {
             Mono F{1.0, 0.0, -kBig};
@@ -758,24 +787,77 @@ struct OptionPde {
         if (step == 0) {
             const double half_dt = 0.5 * dt;
             c0.invalidate();
-            bool ok = stage(0.0 + half_dt, half_dt, un, un, c0, u);
+            bool ok = stage(g_right(0.0 + half_dt), half_dt, un, un, c0, u);
 #pragma unroll
             for (int k = 0; k < M; ++k) mid[k] = u[k];
-            ok = ok && stage(0.0 + dt, half_dt, mid, mid, c0, u);
+            ok = ok && stage(g_right(0.0 + dt), half_dt, mid, mid, c0, u);
             c0.invalidate();
             return ok;
         }
         const double t = static_cast<double>(step) * dt;
-        const double t_next = static_cast<double>(step + 1) * dt;
+        const double disc = put ? 0.0 : exp(-r * t);
         const double w1 = tr.w1(dt);
         stage1_rhs(w1, un, rhs);
-        if (!stage(t + tr.gamma * dt, w1, rhs, un, c0, u)) return false;
+        if (!stage(g_right_d(disc * e_g), w1, rhs, un, c0, u)) return false;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             mid[k] = u[k];
             rhs[k] = fma(tr.alpha, u[k], tr.beta * un[k]);
         }
-        return stage(t_next, tr.w2(dt), rhs, mid, c1, u);
+        return stage(g_right_d(disc * e_dt), tr.w2(dt), rhs, mid, c1, u);
+    }
+
+    // Lock pattern eliminate() will use for this guess.
+    __device__ __forceinline__ uint32_t lock_mask(const double (&guess)[M]) const {
+        uint32_t mask = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k)
+            if (((lockable >> k) & 1u) && (guess[k] - psi[k] < kExerciseTol)) mask |= (1u << k);
+        return mask;
+    }
+
+    // Apply the accumulated denominator-adjoint sources of a stage kind with
+    // the kind's current factorization c: db_i = hB_i db_{i+1} + H_i (node-
+    // decreasing scan), then gd += -w db, gl += w db c'_{i-1},
+    // gu += -w nlo_{i+1} db_{i+1} / den_i on the free interior rows.
+    __device__ __forceinline__ void flush(const StageCache<M>& c, DenAcc<M>& a, double (&gl)[M], double (&gd)[M],
+                                          double (&gu)[M]) const {
+        if (!a.active) return;
+        const double w = a.w;
+        double nlo[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) nlo[k] = ((c.mask >> k) & 1u) ? 0.0 : w * jl[k];
+        const double nlo_top = from_above(nlo[0]);
+        double nlx[M], hB[M];
+        Aff H{0.0, 1.0};
+#pragma unroll
+        for (int k = M - 1; k >= 0; --k) {
+            nlx[k] = (k == M - 1) ? nlo_top : nlo[k < M - 1 ? k + 1 : 0];
+            hB[k] = -(c.cp[k] * c.iv[k]) * nlx[k];
+            H = Aff{fma(hB[k], H.A, a.h[k]), hB[k] * H.B};
+        }
+        H = scan_prefix<K>(H);
+        double hin = from_above(H.A);
+        double db[M];
+#pragma unroll
+        for (int k = M - 1; k >= 0; --k) {
+            db[k] = fma(hB[k], hin, a.h[k]);
+            hin = db[k];
+        }
+        const double cp_in = from_below(c.cp[M - 1]);
+        const double db_top = from_above(db[0]);
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            const double cpv = (k == 0) ? cp_in : c.cp[k > 0 ? k - 1 : 0];
+            const double dbn = (k == M - 1) ? db_top : db[k < M - 1 ? k + 1 : 0];
+            if (i >= 1 && i <= n - 2 && !((c.mask >> k) & 1u)) {
+                gl[k] = fma(w, db[k] * cpv, gl[k]);
+                gd[k] = fma(-w, db[k], gd[k]);
+                gu[k] = fma(-w, nlx[k] * dbn * c.iv[k], gu[k]);
+            }
+        }
+        a.clear();
     }
 
     // Reverse of one stage with the exercise sets frozen (the locked rows and
@@ -783,20 +865,25 @@ struct OptionPde {
     // rhs and guess, its output uo (after the boundary values) and the adjoint
     // ubar of that output. Out: rbf, the adjoint of the right-hand side on free
     // rows (zero elsewhere). Adds the stage's terms to the per-node adjoints of
-    // the Jacobian bands (gl, gd, gu) and to rbc (d/dr of the call boundary).
+    // the Jacobian bands (gl, gd, gu) and to rbc (d/dr of the call boundary);
+    // the denominator-adjoint part goes to the stage kind's accumulator (a
+    // change of the kind's factorization flushes it first).
     //   back substitution reversed: t_i = ubar_i + e_{i-1} t_{i-1},
     //     e_i = clamped_i ? 0 : -c'_i, g_i = clamped_i ? 0 : t_i     (upward)
     //   elimination reversed: delta_i = (g_i + nlo_{i+1} delta_{i+1}) / den_i,
     //     denbar_i = -delta_i d'_i - c'_i cbar'_i / den_i,
     //     cbar'_i = -g_i u_{i+1} + nlo_{i+1} denbar_{i+1}             (downward)
-    __device__ __forceinline__ bool stage_reverse(double t, double w, const double (&rhs)[M], const double (&guess)[M],
-                                                  StageCache<M>& c, const double (&uo)[M], const double (&ubar)[M],
-                                                  double (&rbf)[M], double (&gl)[M], double (&gd)[M], double (&gu)[M],
+    // t and disc = e^{-r t}: the stage time and its discount factor.
+    __device__ __forceinline__ bool stage_reverse(double t, double disc, double w, const double (&rhs)[M],
+                                                  const double (&guess)[M], StageCache<M>& c, DenAcc<M>& acc,
+                                                  const double (&uo)[M], const double (&ubar)[M], double (&rbf)[M],
+                                                  double (&gl)[M], double (&gd)[M], double (&gu)[M],
                                                   double& rbc) const {
-        double nlo[M], dp[M], gr;
+        if ((__ballot(lock_mask(guess) != c.mask) & gmask) != 0) flush(c, acc, gl, gd, gu);
+        double nlo[M], dp[M];
         uint32_t mask;
-        if (!eliminate(t, w, rhs, guess, c, nlo, dp, mask, gr)) return false;
-        const double dgr = g_right_dr(t);
+        if (!eliminate(g_right_d(disc), w, rhs, guess, c, nlo, dp, mask)) return false;
+        const double dgr = g_right_dr_d(t, disc);
         // the solve's own value at n-1 (the output carries the boundary value)
         double uth[M];
 #pragma unroll
@@ -850,46 +937,24 @@ struct OptionPde {
             dl[k] = c.iv[k] * fma(nlx[k], din, g[k]);
             din = dl[k];
         }
-        // denominator-adjoint scan (node-decreasing)
-        double hA[M], hB[M];
-        Aff H{0.0, 1.0};
-#pragma unroll
-        for (int k = M - 1; k >= 0; --k) {
-            const double ci = c.cp[k] * c.iv[k];
-            hA[k] = fma(ci * g[k], unx[k], -dl[k] * dp[k]);
-            hB[k] = -ci * nlx[k];
-            H = Aff{fma(hB[k], H.A, hA[k]), hB[k] * H.B};
-        }
-        H = scan_prefix<K>(H);
-        double hin = from_above(H.A);
-        double db[M];
-#pragma unroll
-        for (int k = M - 1; k >= 0; --k) {
-            db[k] = fma(hB[k], hin, hA[k]);
-            hin = db[k];
-        }
-        // band adjoints of the free interior rows
-        const double cp_in = from_below(c.cp[M - 1]);
+        // denominator-adjoint sources (accumulated) and the per-stage band terms
         const double dp_in = from_below(dp[M - 1]);
-        const double db_top = from_above(db[0]);
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             const int i = base + k;
-            const double cpv = (k == 0) ? cp_in : c.cp[k > 0 ? k - 1 : 0];
+            const double ci = c.cp[k] * c.iv[k];
+            acc.h[k] += fma(ci * g[k], unx[k], -dl[k] * dp[k]);
             const double dpv = (k == 0) ? dp_in : dp[k > 0 ? k - 1 : 0];
-            const double dbn = (k == M - 1) ? db_top : db[k < M - 1 ? k + 1 : 0];
-            const double nlob = fma(db[k], cpv, dl[k] * dpv);
-            const double cpb = fma(nlx[k], dbn, -g[k] * unx[k]);
-            const double ubb = cpb * c.iv[k];
             const bool free_row = i >= 1 && i <= n - 2 && !((mask >> k) & 1u);
             if (free_row) {
-                gl[k] = fma(w, nlob, gl[k]);
-                gd[k] = fma(-w, db[k], gd[k]);
-                gu[k] = fma(-w, ubb, gu[k]);
+                gl[k] = fma(w, dl[k] * dpv, gl[k]);
+                gu[k] = fma(w, c.iv[k] * (g[k] * unx[k]), gu[k]);
             }
             rbf[k] = free_row ? dl[k] : 0.0;
             if (k == k_right) rbc = fma(dl[k], dgr, rbc);
         }
+        acc.w = w;
+        acc.active = true;
         return true;
     }
 

@@ -135,7 +135,10 @@ __device__ __forceinline__ void solve_one(const PdeJob* __restrict__ jobs, int j
     const double x_first = valid ? xs[0] : 0.0;
     const double x_last = valid ? xs[n - 1] : 0.0;
     const double g_left = put ? smax(1.0 - exp(x_first), 0.0) : 0.0;
-    auto g_right = [&](double t) { return put ? 0.0 : exp(x_last) - exp(-r * t); };
+    const double ex_last = put ? 0.0 : exp(x_last);
+    // call boundary e^{x_{n-1}} - e^{-r t} from the discount factor e^{-r t}
+    auto g_right_d = [&](double disc) { return put ? 0.0 : ex_last - disc; };
+    auto g_right = [&](double t) { return put ? 0.0 : ex_last - exp(-r * t); };
 
     // ---- step count (TimeDomain::from_n_steps; auto: grid_spec_types.cpp:90-99) ----
     int64_t n_steps = job.n_time;
@@ -173,9 +176,8 @@ __device__ __forceinline__ void solve_one(const PdeJob* __restrict__ jobs, int j
     // One projected implicit stage (solve_implicit_stage_projected,
     // pde_solver.hpp:525-729): bands, Dirichlet rows, deep-ITM locks,
     // projected Thomas, boundary values. rhs/guess in, u out.
-    auto stage = [&](double t, double w, const double (&rhs)[M], const double (&guess)[M], double (&cp)[M],
+    auto stage = [&](double gr, double w, const double (&rhs)[M], const double (&guess)[M], double (&cp)[M],
                      double (&iv)[M], uint32_t& cmask, bool& csing, bool& cmono) -> bool {
-        const double gr = g_right(t);
         double nlo[M], rb[M];
         uint32_t mask = 0;
 #pragma unroll
@@ -302,11 +304,11 @@ __device__ __forceinline__ void solve_one(const PdeJob* __restrict__ jobs, int j
         const double t_mid = t0 + half_dt;
 #pragma unroll
         for (int k = 0; k < M; ++k) { up[k] = u[k]; rhs[k] = up[k]; }
-        ok = stage(t_mid, half_dt, rhs, up, cp0, iv0, cmask0, csing0, cmono0);
+        ok = stage(g_right(t_mid), half_dt, rhs, up, cp0, iv0, cmask0, csing0, cmono0);
         if (ok) {
 #pragma unroll
             for (int k = 0; k < M; ++k) { up[k] = u[k]; rhs[k] = up[k]; }
-            ok = stage(t_next, half_dt, rhs, up, cp0, iv0, cmask0, csing0, cmono0);
+            ok = stage(g_right(t_next), half_dt, rhs, up, cp0, iv0, cmask0, csing0, cmono0);
         }
         cmask0 = ~0u;  // the Euler weight differs from stage 1's
     }
@@ -319,8 +321,15 @@ __device__ __forceinline__ void solve_one(const PdeJob* __restrict__ jobs, int j
         const double beta = -(omg * omg) / den;
         const double w2 = (1.0 - gamma) * dt / (2.0 - gamma);
         double t = static_cast<double>(1) * dt;
+        // discount factors of the stage times: e^{-r t_s} carried from step to
+        // step (exact every 16 steps), times e^{-r gamma dt} for stage 1 and
+        // e^{-r dt} for stage 2 — no exp per stage
+        const double e_dt = put ? 0.0 : exp(-r * dt);
+        const double e_g = put ? 0.0 : exp(-r * (gamma * dt));
+        double disc = 0.0;
         for (int64_t step = 1; step < n_steps; ++step) {
             const double t_next = static_cast<double>(step + 1) * dt;
+            if (!put && ((step - 1) & 15) == 0) disc = exp(-r * t);
 #pragma unroll
             for (int k = 0; k < M; ++k) up[k] = u[k];
             // stage 1: rhs = u^n + w1 L(u^n), guess u^n (pde_solver.hpp:342-362);
@@ -337,13 +346,14 @@ __device__ __forceinline__ void solve_one(const PdeJob* __restrict__ jobs, int j
                     rhs[k] = fma(w1, lu, up[k]);  // jl = jd = ju = 0 off the interior: L = 0
                 }
             }
-            ok = stage(t + gamma * dt, w1, rhs, up, cp0, iv0, cmask0, csing0, cmono0);
+            ok = stage(g_right_d(disc * e_g), w1, rhs, up, cp0, iv0, cmask0, csing0, cmono0);
             if (!ok) break;
             // stage 2: rhs = alpha u* + beta u^n, guess u* (pde_solver.hpp:370-395)
             double ustar[M];
 #pragma unroll
             for (int k = 0; k < M; ++k) { ustar[k] = u[k]; rhs[k] = fma(alpha, u[k], beta * up[k]); }
-            ok = stage(t_next, w2, rhs, ustar, cp1, iv1, cmask1, csing1, cmono1);
+            disc *= e_dt;
+            ok = stage(g_right_d(disc), w2, rhs, ustar, cp1, iv1, cmask1, csing1, cmono1);
             if (!ok) break;
             t = t_next;
         }

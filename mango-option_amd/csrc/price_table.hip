@@ -59,6 +59,7 @@ __global__ __launch_bounds__(kSolveBlock) void table_snapshot_kernel(const PdeJo
     const int64_t N = P.valid ? P.steps(job) : 0;
     bool ok = P.valid && N > 0 && n <= row;
     const double dt = ok ? job.maturity / static_cast<double>(N) : 0.0;
+    P.set_dt(dt);
     const int32_t* st = steps + static_cast<size_t>(j) * max_snap;
     const int ns = n_snap[j];
     if (ok) {
