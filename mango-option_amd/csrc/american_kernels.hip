@@ -436,10 +436,12 @@ __global__ __launch_bounds__(kSolveBlock) void american_solve_kernel(const PdeJo
     const int slot = threadIdx.x / K;
     const int lane = threadIdx.x % K;
     if constexpr (!QUEUE) {
-        const int j = blockIdx.x * GPB + slot;
-        if (j >= n_jobs) return;  // the whole group leaves; synchronization is per wavefront
-        solve_one<K, M>(jobs, job_list ? job_list[j] : j, lane, s_x[slot], s_y[slot], s_cp[slot], s_c[slot],
-                        greeks_out, status_out, state);
+        // grid-stride over the jobs (a launch may hold fewer groups than jobs
+        // when the job count is only known on the device); the loop condition
+        // is uniform within a group, whose synchronization is per wavefront
+        for (int j = blockIdx.x * GPB + slot; j < n_jobs; j += gridDim.x * GPB)
+            solve_one<K, M>(jobs, job_list ? job_list[j] : j, lane, s_x[slot], s_y[slot], s_cp[slot], s_c[slot],
+                            greeks_out, status_out, state);
     } else {
         for (;;) {
             // Every lane takes part in the fetch (only the group's lane 0
@@ -522,9 +524,13 @@ __global__ __launch_bounds__(64) void plan_chains_kernel(const DeviceOptionRecor
         const int64_t other = __shfl_xor(gnx, o, 64);
         gnx = gnx > other ? gnx : other;
     }
-    // greedy first-match grouping on one lane (order-dependent, like the reference)
     constexpr double TOL = 1e-10;
-    if (lane == 0) {
+    auto same_pde = [&](const DeviceOptionRecord& p, const DeviceOptionRecord& o) {
+        return fabs(p.volatility - o.volatility) < TOL && fabs(p.rate - o.rate) < TOL &&
+               fabs(p.dividend_yield - o.dividend_yield) < TOL && fabs(p.maturity - o.maturity) < TOL &&
+               p.option_type == o.option_type;
+    };
+    auto shared_grid = [&]() {
         GridDesc shared;
         if (plan.mode == 1) {
             shared = plan.spec;
@@ -536,16 +542,83 @@ __global__ __launch_bounds__(64) void plan_chains_kernel(const DeviceOptionRecor
                 shared.concentration = plan.accuracy.alpha; shared.n_clusters = 0;
             }
         }
+        return shared;
+    };
+    auto make_job = [&](const DeviceOptionRecord& o, const GridDesc& g, int32_t slot) {
+        PdeJob jb;
+        jb.maturity = o.maturity; jb.volatility = o.volatility; jb.rate = o.rate;
+        jb.dividend_yield = o.dividend_yield;
+        jb.spot = 1.0; jb.strike = 1.0; jb.c_t = plan.accuracy.c_t; jb.pad0 = 0.0;
+        jb.grid = g;
+        jb.option_type = o.option_type;
+        jb.n_time = plan.n_time;
+        jb.max_time_steps = plan.accuracy.max_time_steps;
+        jb.min_time_steps = 0;
+        jb.out_index = slot;
+        jb.flags = kJobWriteState;
+        jobs[slot] = jb;
+    };
+    if (e - b <= 64) {
+        // Segments of up to one wavefront: lane i holds option i and the mask
+        // of earlier options whose PDE parameters match within TOL; lane 0
+        // walks the options in order, each joining the first earlier group
+        // representative it matches (the reference's first-match grouping),
+        // and the representatives write their jobs in parallel.
+        __shared__ DeviceOptionRecord s_opt[64];
+        __shared__ unsigned long long s_match[64];
+        __shared__ int32_t s_rep[64];
+        const int cnt = static_cast<int>(e - b);
+        if (lane < cnt) s_opt[lane] = opts[b + lane];
+        __syncthreads();
+        unsigned long long match = 0;
+        if (lane < cnt) {
+            const DeviceOptionRecord o = s_opt[lane];
+            for (int j = 0; j < lane; ++j)
+                if (same_pde(s_opt[j], o)) match |= 1ull << j;
+        }
+        s_match[lane] = match;
+        __syncthreads();
+        if (lane == 0) {
+            unsigned long long reps = 0;
+            for (int i = 0; i < cnt; ++i) {
+                const unsigned long long m = s_match[i] & reps;
+                if (m) {
+                    s_rep[i] = __ffsll(static_cast<long long>(m)) - 1;
+                } else {
+                    s_rep[i] = i;
+                    reps |= 1ull << i;
+                }
+            }
+        }
+        __syncthreads();
+        const bool is_rep = lane < cnt && s_rep[lane] == lane;
+        const unsigned long long rep_mask = __ballot(is_rep);
+        int32_t base = 0;
+        if (lane == 0) base = atomicAdd(job_count, __popcll(rep_mask));
+        base = __shfl(base, 0, 64);
+        if (lane < cnt) {
+            const int r = s_rep[lane];
+            const int32_t slot = base + __popcll(rep_mask & ((1ull << r) - 1ull));
+            if (is_rep) make_job(s_opt[lane], shared_grid(), slot);
+            ChainOption c;
+            c.spot = s_opt[lane].spot;
+            c.strike = s_opt[lane].strike;
+            c.group = slot;
+            c.out_index = static_cast<int32_t>(b + lane);
+            copts[b + lane] = c;
+        }
+        return;
+    }
+    // longer segments: greedy first-match grouping on one lane
+    if (lane == 0) {
+        const GridDesc shared = shared_grid();
         int32_t n_groups = 0;
         for (int64_t i = b; i < e; ++i) {
             const DeviceOptionRecord o = opts[i];
             int32_t r = static_cast<int32_t>(i);
             for (int64_t j = b; j < i; ++j) {
                 if (rep[j] != j) continue;
-                const DeviceOptionRecord p = opts[j];
-                if (fabs(p.volatility - o.volatility) < TOL && fabs(p.rate - o.rate) < TOL &&
-                    fabs(p.dividend_yield - o.dividend_yield) < TOL && fabs(p.maturity - o.maturity) < TOL &&
-                    p.option_type == o.option_type) {
+                if (same_pde(opts[j], o)) {
                     r = static_cast<int32_t>(j);
                     break;
                 }
@@ -557,18 +630,7 @@ __global__ __launch_bounds__(64) void plan_chains_kernel(const DeviceOptionRecor
         int32_t ord = 0;
         for (int64_t i = b; i < e; ++i) {
             if (rep[i] != i) continue;
-            const DeviceOptionRecord o = opts[i];
-            PdeJob j;
-            j.maturity = o.maturity; j.volatility = o.volatility; j.rate = o.rate; j.dividend_yield = o.dividend_yield;
-            j.spot = 1.0; j.strike = 1.0; j.c_t = plan.accuracy.c_t; j.pad0 = 0.0;
-            j.grid = shared;
-            j.option_type = o.option_type;
-            j.n_time = plan.n_time;
-            j.max_time_steps = plan.accuracy.max_time_steps;
-            j.min_time_steps = 0;
-            j.out_index = base + ord;
-            j.flags = kJobWriteState;
-            jobs[base + ord] = j;
+            make_job(opts[i], shared, base + ord);
             rep[i] = -(base + ord) - 1;  // representative: its group slot
             ++ord;
         }
@@ -729,6 +791,7 @@ int launch_variant(const PdeJob* d_jobs, int n_jobs, const int32_t* d_n_jobs, Gr
         hipLaunchKernelGGL((american_solve_kernel<K, M, true>), dim3(blocks), dim3(kSolveBlock), 0, stream, d_jobs,
                            n_jobs, d_n_jobs, job_list, d_list_offset, queue, d_greeks, d_status, state);
     } else {
+        if (max_blocks > 0) blocks = std::min(blocks, max_blocks);
         if (blocks <= 0) return 0;
         hipLaunchKernelGGL((american_solve_kernel<K, M, false>), dim3(blocks), dim3(kSolveBlock), 0, stream, d_jobs,
                            n_jobs, d_n_jobs, job_list, d_list_offset, queue, d_greeks, d_status, state);

@@ -67,7 +67,8 @@ struct SolveVariant {
 // d_n_jobs (optional, device): the kernel processes min(n_jobs, *d_n_jobs) jobs.
 // job_list (optional, device): slot -> job index. queue (optional, device,
 // zeroed by the caller): persistent work-queue form with at most max_blocks
-// (0 = resident capacity) one-wavefront blocks.
+// (0 = resident capacity) one-wavefront blocks. Without a queue, max_blocks
+// > 0 caps the grid and the groups stride over the jobs.
 int launch_american_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs,
                           Greeks* d_greeks, int32_t* d_status, const StateOut& state,
                           hipStream_t stream, const int32_t* d_n_jobs = nullptr,

@@ -504,7 +504,7 @@ std::vector<std::pair<hipEvent_t, hipEvent_t>> g_events;
 
 int timed_solve_ex(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, Greeks* d_greeks, int32_t* d_status,
                    const StateOut& state, hipStream_t stream, const int32_t* d_n_jobs, const int32_t* job_list,
-                   int32_t* queue, const int32_t* d_list_offset) {
+                   int32_t* queue, const int32_t* d_list_offset, int max_blocks = 0) {
     hipEvent_t a = nullptr, b = nullptr;
     if (g_timing) {
         (void)hipEventCreate(&a);
@@ -512,7 +512,7 @@ int timed_solve_ex(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, Gree
         (void)hipEventRecord(a, stream);
     }
     const int rc = launch_american_solve(v, d_jobs, n_jobs, d_greeks, d_status, state, stream, d_n_jobs, job_list,
-                                         queue, 0, d_list_offset);
+                                         queue, max_blocks, d_list_offset);
     if (g_timing) {
         (void)hipEventRecord(b, stream);
         g_events.push_back({a, b});
@@ -605,7 +605,12 @@ MangoStatus device_chains(const MangoDeviceOption* d_opts, uint64_t n, const int
     // about two PDEs (one per option type) per chain segment
     const SolveVariant var = 2 * static_cast<int64_t>(n_segments) <= kLatencyJobs ? pick_latency_variant(n_max)
                                                                                    : pick_variant(n_max);
-    ok = ok && timed_solve(var, g_chain.jobs, ni, nullptr, g_chain.gstatus, so, stream, g_chain.job_count) == 0;
+    // the job count is known only on the device: a grid sized for about two
+    // PDEs per segment, its groups striding over however many there are
+    const int groups = static_cast<int>(std::min<int64_t>(ni, std::max<int64_t>(2 * n_segments, 1)));
+    const int gpb = 64 / var.lanes;  // groups per one-wavefront block
+    ok = ok && timed_solve_ex(var, g_chain.jobs, ni, nullptr, g_chain.gstatus, so, stream, g_chain.job_count, nullptr,
+                              nullptr, nullptr, (groups + gpb - 1) / gpb) == 0;
     ok = ok && launch_chain_epilogue(g_chain.copts, ni, so, g_chain.gstatus, reinterpret_cast<Greeks*>(d_out),
                                      d_status, stream) == 0;
     if (!ok) {
