@@ -19,7 +19,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kSolveBlock = 64;  // one wavefront per block
-constexpr double kBig = 1e300;
+constexpr double kBig = 0x1p1000;  // power of two: zero low dword (see dpp)
 constexpr double kSingularTol = 1e-15;   // ThomasConfig::singularity_tol
 constexpr double kDeepItmFraction = 0.95; // pde_solver.hpp:635
 constexpr double kExerciseTol = 1e-8;     // pde_solver.hpp:636
@@ -247,9 +247,23 @@ constexpr int kWaveShr1 = 0x138;
 constexpr int kRowBcast15 = 0x142;
 constexpr int kRowBcast31 = 0x143;
 
+// Lane move with identity `old` for lanes the move does not write. A double
+// moves as two dwords; where every lane is written (row mask 0xf) a dword
+// whose identity is zero comes from bound_ctrl, which writes 0 for sources
+// outside the row or wave, so only nonzero identity dwords are materialized
+// in the destination first. Every identity used here (0, 1, +-kBig) has a
+// zero low dword.
 template <int CTRL, int ROWS = 0xf>
 __device__ __forceinline__ double dpp(double old, double v) {
-    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xf, false);
+    const long long ob = __double_as_longlong(old);
+    const long long vb = __double_as_longlong(v);
+    const int o_lo = static_cast<int>(ob & 0xffffffffll), o_hi = static_cast<int>(ob >> 32);
+    const int v_lo = static_cast<int>(vb & 0xffffffffll), v_hi = static_cast<int>(vb >> 32);
+    const int r_lo = (ROWS == 0xf && o_lo == 0) ? __builtin_amdgcn_mov_dpp(v_lo, CTRL, 0xf, 0xf, true)
+                                                : __builtin_amdgcn_update_dpp(o_lo, v_lo, CTRL, ROWS, 0xf, false);
+    const int r_hi = (ROWS == 0xf && o_hi == 0) ? __builtin_amdgcn_mov_dpp(v_hi, CTRL, 0xf, 0xf, true)
+                                                : __builtin_amdgcn_update_dpp(o_hi, v_hi, CTRL, ROWS, 0xf, false);
+    return __longlong_as_double((static_cast<long long>(r_hi) << 32) | static_cast<unsigned int>(r_lo));
 }
 // Lanes whose source is outside the row read 0 (no identity fill needed).
 template <int CTRL>
