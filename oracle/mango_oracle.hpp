@@ -38,6 +38,7 @@
 #include <cstdint>
 #include <limits>
 #include <optional>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -45,8 +46,94 @@ namespace oracle {
 
 enum class OptionType { CALL = 0, PUT = 1 };
 
-// PricingParams restricted to a constant rate and no discrete dividends
-// (src/option/option_spec.hpp:185-192, 262-281).
+// ---------------------------------------------------------------------------
+// Yield curve: log-linear discount interpolation, piecewise-constant forward
+// rates (src/option/yield_curve.hpp:19-180)
+// ---------------------------------------------------------------------------
+struct TenorPoint {
+    double tenor;
+    double log_discount;
+};
+
+struct YieldCurve {
+    std::vector<TenorPoint> pts;  // sorted, pts[0].tenor == 0
+
+    // YieldCurve::from_points (:47-79): sort, t=0 first with ln D = 0, tenors
+    // strictly increasing by more than 1e-10. Empty message on success.
+    static std::optional<YieldCurve> from_points(std::vector<TenorPoint> points, std::string* why = nullptr) {
+        auto fail = [&](const char* m) { if (why) *why = m; return std::optional<YieldCurve>{}; };
+        if (points.empty()) return fail("Empty points vector");
+        std::sort(points.begin(), points.end(),
+                  [](const TenorPoint& a, const TenorPoint& b) { return a.tenor < b.tenor; });
+        if (points[0].tenor != 0.0) return fail("First point must have t=0");
+        if (std::abs(points[0].log_discount) > 1e-10) return fail("log_discount at t=0 must be 0");
+        for (size_t i = 1; i < points.size(); ++i)
+            if (points[i].tenor <= points[i - 1].tenor + 1e-10) return fail("Tenors must be strictly increasing");
+        YieldCurve c;
+        c.pts = std::move(points);
+        return c;
+    }
+    // YieldCurve::flat (:37-44)
+    static YieldCurve flat(double rate) {
+        YieldCurve c;
+        c.pts = {{0.0, 0.0}, {100.0, -rate * 100.0}};
+        return c;
+    }
+    // index of the last tenor <= t (upper_bound - 1), t > 0
+    size_t bracket(double t) const {
+        auto it = std::upper_bound(pts.begin(), pts.end(), t,
+                                   [](double v, const TenorPoint& q) { return v < q.tenor; });
+        return static_cast<size_t>(std::distance(pts.begin(), it));
+    }
+    double rate_between(size_t i) const {  // (:170-178)
+        if (i + 1 >= pts.size()) return 0.0;
+        const double dt = pts[i + 1].tenor - pts[i].tenor;
+        if (dt <= 0.0) return 0.0;
+        return -(pts[i + 1].log_discount - pts[i].log_discount) / dt;
+    }
+    double rate(double t) const {  // (:104-118)
+        if (pts.size() < 2) return 0.0;
+        if (t <= 0.0) return rate_between(0);
+        const size_t ub = bracket(t);
+        if (ub == 0) return rate_between(0);
+        if (ub == pts.size()) return rate_between(pts.size() - 2);
+        return rate_between(ub - 1);
+    }
+    double log_discount(double t) const {  // (:133-156)
+        if (pts.size() < 2) return 0.0;
+        if (t <= 0.0) return 0.0;
+        const size_t ub = bracket(t);
+        if (ub == 0) return 0.0;
+        if (ub == pts.size()) {
+            const auto& last = pts.back();
+            const auto& prev = pts[pts.size() - 2];
+            const double r = -(last.log_discount - prev.log_discount) / (last.tenor - prev.tenor);
+            return last.log_discount - r * (t - last.tenor);
+        }
+        const auto& right = pts[ub];
+        const auto& left = pts[ub - 1];
+        const double a = (t - left.tenor) / (right.tenor - left.tenor);
+        return left.log_discount + a * (right.log_discount - left.log_discount);
+    }
+    double discount(double t) const { return std::exp(log_discount(t)); }  // (:121-123)
+    bool operator==(const YieldCurve& o) const {  // (:159-168)
+        if (pts.size() != o.pts.size()) return false;
+        for (size_t i = 0; i < pts.size(); ++i)
+            if (std::abs(pts[i].tenor - o.pts[i].tenor) > 1e-12 ||
+                std::abs(pts[i].log_discount - o.pts[i].log_discount) > 1e-12)
+                return false;
+        return true;
+    }
+};
+
+struct Dividend {
+    double calendar_time;
+    double amount;
+};
+
+// PricingParams (src/option/option_spec.hpp:185-192, 262-281): a constant
+// rate unless `curve` is set (RateSpec = double | YieldCurve), plus discrete
+// cash dividends at calendar times.
 struct PricingParams {
     double spot = 0.0;
     double strike = 0.0;
@@ -55,17 +142,65 @@ struct PricingParams {
     double dividend_yield = 0.0;
     double volatility = 0.0;
     OptionType type = OptionType::PUT;
+    std::optional<YieldCurve> curve;
+    std::vector<Dividend> dividends;
 };
+
+// make_rate_fn (option_spec.hpp:50-64): the short rate at time-to-expiry tau
+// is the curve's forward rate at calendar time max(0, T - tau).
+inline double rate_at(const PricingParams& p, double tau) {
+    if (!p.curve) return p.rate;
+    return p.curve->rate(std::max(0.0, p.maturity - tau));
+}
+// make_forward_discount_fn (option_spec.hpp:75-91): D(T) / D(max(0, T - tau)).
+inline double forward_discount(const PricingParams& p, double tau) {
+    if (!p.curve) return std::exp(-p.rate * tau);
+    const double s = std::max(0.0, p.maturity - tau);
+    const double d_T = p.curve->discount(p.maturity);
+    const double d_s = p.curve->discount(s);
+    return d_T / d_s;
+}
 
 // validate_option_spec + validate_pricing_params (src/option/option_spec.cpp:8-50, 117-155).
 inline bool validate_pricing_params(const PricingParams& p) {
     if (p.spot <= 0.0 || !std::isfinite(p.spot)) return false;
     if (p.strike <= 0.0 || !std::isfinite(p.strike)) return false;
     if (p.maturity <= 0.0 || !std::isfinite(p.maturity)) return false;
-    if (!std::isfinite(p.rate)) return false;
+    if (!p.curve && !std::isfinite(p.rate)) return false;
     if (p.dividend_yield < 0.0 || !std::isfinite(p.dividend_yield)) return false;
     if (p.volatility <= 0.0 || !std::isfinite(p.volatility)) return false;
+    for (const auto& d : p.dividends) {
+        if (d.calendar_time < 0.0 || d.calendar_time > p.maturity) return false;
+        if (d.amount < 0.0) return false;
+        if (!std::isfinite(d.calendar_time) || !std::isfinite(d.amount)) return false;
+    }
     return true;
+}
+
+// filter_and_merge_dividends (src/option/dividend_utils.hpp:14-40): keep
+// 0 < t < T with a positive amount, sort by time, sum same-date amounts.
+inline std::vector<Dividend> filter_and_merge_dividends(const std::vector<Dividend>& divs, double T) {
+    std::vector<Dividend> f;
+    for (const auto& d : divs)
+        if (d.calendar_time > 0.0 && d.calendar_time < T && d.amount > 0.0) f.push_back(d);
+    std::sort(f.begin(), f.end(), [](const Dividend& a, const Dividend& b) { return a.calendar_time < b.calendar_time; });
+    std::vector<Dividend> m;
+    for (const auto& d : f) {
+        if (!m.empty() && std::abs(m.back().calendar_time - d.calendar_time) < 1e-12) m.back().amount += d.amount;
+        else m.push_back(d);
+    }
+    return m;
+}
+
+// Dividend times as times-to-expiry strictly inside (0, T)
+// (grid_spec_types.cpp:101-108; american_option_batch.cpp:448-457).
+inline std::vector<double> mandatory_taus(const PricingParams& p) {
+    std::vector<double> out;
+    for (const auto& d : p.dividends) {
+        const double tau = p.maturity - d.calendar_time;
+        if (tau > 0.0 && tau < p.maturity) out.push_back(tau);
+    }
+    return out;
 }
 
 // ---------------------------------------------------------------------------
@@ -316,18 +451,59 @@ struct Spacing {
 };
 
 // ---------------------------------------------------------------------------
-// Time domain (src/pde/core/time_domain.hpp:36-151), uniform steps only
+// Time domain (src/pde/core/time_domain.hpp:36-151): uniform steps, or
+// explicit points when mandatory times split the interval
 // ---------------------------------------------------------------------------
 struct TimeDomain {
     double t_start = 0.0, t_end = 1.0, dt = 0.01;
     size_t n_steps = 100;
+    std::vector<double> points;  // explicit points (empty = uniform)
     static TimeDomain from_n_steps(double t0, double t1, size_t n) {
         TimeDomain td;
         td.t_start = t0; td.t_end = t1; td.n_steps = n;
         td.dt = (t1 - t0) / static_cast<double>(n);
         return td;
     }
-    double time_point(size_t i) const { return t_start + static_cast<double>(i) * dt; }
+    // with_mandatory_points (:55-107): each interval between consecutive
+    // mandatory points (strictly inside, deduplicated at 1e-14) is cut into
+    // ceil(len / dt) equal steps.
+    static TimeDomain with_mandatory_points(double t0, double t1, double dt, std::vector<double> mandatory) {
+        std::vector<double> breaks;
+        for (double t : mandatory)
+            if (t > t0 + 1e-14 && t < t1 - 1e-14) breaks.push_back(t);
+        std::sort(breaks.begin(), breaks.end());
+        breaks.erase(std::unique(breaks.begin(), breaks.end(),
+                                 [](double a, double b) { return std::abs(a - b) < 1e-14; }),
+                     breaks.end());
+        if (breaks.empty()) {
+            const size_t n = static_cast<size_t>(std::ceil((t1 - t0) / dt));
+            return from_n_steps(t0, t1, n);
+        }
+        std::vector<double> bounds{t0};
+        for (double b : breaks) bounds.push_back(b);
+        bounds.push_back(t1);
+        std::vector<double> pts{t0};
+        for (size_t sgi = 0; sgi + 1 < bounds.size(); ++sgi) {
+            const double a = bounds[sgi], len = bounds[sgi + 1] - a;
+            const size_t n_sub = std::max<size_t>(1, static_cast<size_t>(std::ceil(len / dt)));
+            const double sub_dt = len / static_cast<double>(n_sub);
+            for (size_t j = 1; j <= n_sub; ++j) pts.push_back(a + static_cast<double>(j) * sub_dt);
+        }
+        TimeDomain td;
+        td.t_start = t0; td.t_end = t1;
+        td.n_steps = pts.size() - 1;
+        td.dt = (t1 - t0) / static_cast<double>(td.n_steps);
+        td.points = std::move(pts);
+        return td;
+    }
+    double time_point(size_t i) const {
+        if (!points.empty()) return points[i];
+        return t_start + static_cast<double>(i) * dt;
+    }
+    double dt_at(size_t step) const {  // (:124-129)
+        if (!points.empty() && step < points.size() - 1) return points[step + 1] - points[step];
+        return dt;
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -345,17 +521,25 @@ struct GridAccuracyParams {
     size_t max_time_steps = 5000;
 };
 
-// grid_spec_types.cpp:37-118 (no discrete dividends)
+// grid_spec_types.cpp:37-118: domain widened for dividend shifts, dividend
+// dates as mandatory time points
 inline std::pair<GridSpec, TimeDomain> estimate_pde_grid(const PricingParams& p,
                                                          const GridAccuracyParams& acc = {}) {
     const double sigma_sqrt_T = p.volatility * std::sqrt(p.maturity);
     const double x0 = std::log(p.spot / p.strike);
-    const double x_min = x0 - acc.n_sigma * sigma_sqrt_T;
+    double x_min = x0 - acc.n_sigma * sigma_sqrt_T;
     const double x_max = x0 + acc.n_sigma * sigma_sqrt_T;
     const double dx_target = p.volatility * std::sqrt(acc.tol);
     size_t Nx = static_cast<size_t>(std::ceil((x_max - x_min) / dx_target));
     Nx = std::clamp(Nx, acc.min_spatial_points, acc.max_spatial_points);
     if (Nx % 2 == 0) Nx++;
+    double max_d_over_k = 0.0;
+    for (const auto& d : p.dividends) {
+        const double tau = p.maturity - d.calendar_time;
+        if (tau > 0.0 && tau < p.maturity) max_d_over_k = std::max(max_d_over_k, d.amount / p.strike);
+    }
+    if (max_d_over_k > 0.0 && std::exp(x_min) > max_d_over_k) x_min = std::log(std::exp(x_min) - max_d_over_k);
+    else if (max_d_over_k > 0.0) x_min -= 1.0;
     std::vector<Cluster> clusters;
     clusters.push_back({x0, acc.alpha, 0.5});
     if (0.0 >= x_min && 0.0 <= x_max) clusters.push_back({0.0, acc.alpha, 1.0});
@@ -367,10 +551,13 @@ inline std::pair<GridSpec, TimeDomain> estimate_pde_grid(const PricingParams& p,
     const double dt = acc.c_t * dx_min;
     size_t Nt = static_cast<size_t>(std::ceil(p.maturity / dt));
     Nt = std::min(Nt, acc.max_time_steps);
-    return {*spec, TimeDomain::from_n_steps(0.0, p.maturity, Nt)};
+    const std::vector<double> mand = mandatory_taus(p);
+    const double dt_capped = std::max(dt, p.maturity / static_cast<double>(acc.max_time_steps));
+    if (mand.empty()) return {*spec, TimeDomain::from_n_steps(0.0, p.maturity, Nt)};
+    return {*spec, TimeDomain::with_mandatory_points(0.0, p.maturity, dt_capped, mand)};
 }
 
-// grid_spec_types.cpp:120-177 (no discrete dividends)
+// grid_spec_types.cpp:120-177
 inline std::pair<GridSpec, TimeDomain> estimate_batch_pde_grid(const std::vector<PricingParams>& ps,
                                                                const GridAccuracyParams& acc = {}) {
     if (ps.empty()) {
@@ -390,7 +577,16 @@ inline std::pair<GridSpec, TimeDomain> estimate_batch_pde_grid(const std::vector
     }
     auto spec = GridSpec::multi_sinh_spaced(gx_min, gx_max, gNx, {{0.0, acc.alpha, 1.0}});
     if (!spec) spec = GridSpec::sinh_spaced(gx_min, gx_max, gNx, acc.alpha);
-    return {*spec, TimeDomain::from_n_steps(0.0, max_T, gNt)};
+    std::vector<double> mand;
+    for (const auto& p : ps)
+        for (const auto& d : p.dividends) {
+            const double tau = p.maturity - d.calendar_time;
+            if (tau > 0.0 && tau < max_T) mand.push_back(tau);
+        }
+    const double g_dt = max_T / static_cast<double>(gNt);
+    const double dt_capped = std::max(g_dt, max_T / static_cast<double>(acc.max_time_steps));
+    if (mand.empty()) return {*spec, TimeDomain::from_n_steps(0.0, max_T, gNt)};
+    return {*spec, TimeDomain::with_mandatory_points(0.0, max_T, dt_capped, mand)};
 }
 
 // ---------------------------------------------------------------------------
@@ -614,6 +810,28 @@ inline bool convert_times_to_indices(const std::vector<double>& times, const Tim
     return true;
 }
 
+// The dividend event callback (american_option.cpp:79-113): u is replaced
+// by its natural spline evaluated at x' = ln(e^x - d), d = D/K; nodes whose
+// shifted spot is at or below 1e-10 K, or left of the grid, take the payoff
+// value of S = 0. A singular spline system leaves u unchanged.
+inline void dividend_shift(const std::vector<double>& x, std::vector<double>& u, double d, double fallback) {
+    if (d <= 0.0) return;
+    const size_t n = x.size();
+    CubicSpline spl;
+    if (!spl.build(x.data(), u.data(), n)) return;
+    const double x_lo = x[0], x_hi = x[n - 1];
+    for (size_t i = 0; i < n; ++i) {
+        const double s_over_k = std::exp(x[i]);
+        const double s_adj = s_over_k - d;
+        if (s_adj > 1e-10) {
+            const double xs = std::log(s_adj);
+            u[i] = (xs < x_lo) ? fallback : spl.eval(std::min(xs, x_hi));
+        } else {
+            u[i] = fallback;
+        }
+    }
+}
+
 // Solve one American option on an explicit grid.
 // AmericanOptionSolver::solve (src/option/american_option.cpp:354-438) driving
 // PDESolver::initialize/solve (src/pde/internal/pde_solver.hpp:96-178).
@@ -642,11 +860,11 @@ inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, cons
     };
 
     const bool put = p.type == OptionType::PUT;
-    const double r = p.rate;
     const double q = p.dividend_yield;
     // BlackScholesPDE (src/pde/operators/black_scholes_pde.hpp:41-46, 72-76, 84-104)
+    // with the rate r(t) of make_rate_fn
     const double hss = 0.5 * p.volatility * p.volatility;
-    auto pde = [&](double d2, double d1, double v) {
+    auto pde = [&](double r, double d2, double d1, double v) {
         const double drift = r - q - hss;
         return hss * d2 + drift * d1 - r * v;
     };
@@ -656,24 +874,28 @@ inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, cons
         psi[i] = put ? std::max(1.0 - std::exp(x[i]), 0.0) : std::max(std::exp(x[i]) - 1.0, 0.0);
     // Dirichlet BCs (src/option/american_option.cpp:192-202, 276-288)
     auto g_left = [&](double) { return put ? std::max(1.0 - std::exp(x[0]), 0.0) : 0.0; };
-    auto g_right = [&](double t) { return put ? 0.0 : std::exp(x[n - 1]) - std::exp(-r * t); };
+    auto g_right = [&](double t) { return put ? 0.0 : std::exp(x[n - 1]) - forward_discount(p, t); };
     auto apply_bc = [&](std::vector<double>& u, double t) { u[0] = g_left(t); u[n - 1] = g_right(t); };
 
     std::vector<double> d2(n, 0.0), d1(n, 0.0);
     // apply_spatial_operator (pde_solver.hpp:322-335; spatial_operator.hpp:74-98)
-    auto apply_L = [&](const std::vector<double>& u, std::vector<double>& Lu) {
+    auto apply_L = [&](double t, const std::vector<double>& u, std::vector<double>& Lu) {
+        const double r = rate_at(p, t);
         std::fill(d2.begin() + 1, d2.begin() + static_cast<std::ptrdiff_t>(n - 1), 0.0);
         std::fill(d1.begin() + 1, d1.begin() + static_cast<std::ptrdiff_t>(n - 1), 0.0);
         sp.second_derivative(u.data(), d2.data(), 1, n - 1);
         sp.first_derivative(u.data(), d1.data(), 1, n - 1);
-        for (size_t i = 1; i < n - 1; ++i) Lu[i] = pde(d2[i], d1[i], u[i]);
+        for (size_t i = 1; i < n - 1; ++i) Lu[i] = pde(r, d2[i], d1[i], u[i]);
         Lu[0] = Lu[n - 1] = 0.0;
     };
-    // Analytic Jacobian bands of L (spatial_operator.hpp:123-164); constant
-    // for a constant rate, so evaluated once (bitwise what the reference
-    // recomputes each stage).
+    // Analytic Jacobian bands of L at rate r (spatial_operator.hpp:123-164);
+    // re-evaluated only when the stage's rate differs from the last one
+    // (bitwise what the reference recomputes each stage).
     std::vector<double> jl(n, 0.0), jd(n, 0.0), ju(n, 0.0);
-    {
+    double bands_r = std::numeric_limits<double>::quiet_NaN();
+    auto bands = [&](double r) {
+        if (r == bands_r) return;
+        bands_r = r;
         const double a = hss;
         const double b = r - q - hss;
         const double c = -r;
@@ -692,7 +914,7 @@ inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, cons
             jd[i] = d2_i + d1_i + c;
             ju[i] = d2_ip1 + d1_ip1;
         }
-    }
+    };
     std::vector<double> lower(n - 1), diag(n), upper(n - 1), rhs_bc(n), lpsi(n), ws(2 * n);
     double psi_max = *std::max_element(psi.begin(), psi.end());
 
@@ -710,6 +932,7 @@ inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, cons
         }
         ++stage_index;
         apply_bc(u, t);
+        bands(rate_at(p, t));
         for (size_t i = 1; i < n - 1; ++i) {
             lower[i - 1] = -w * jl[i];
             diag[i] = 1.0 - w * jd[i];
@@ -734,7 +957,7 @@ inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, cons
                 if (psi[i] > thr && u[i] - psi[i] < 1e-8) { first = i; break; }
             }
             if (first < n - 1) {
-                apply_L(psi, lpsi);
+                apply_L(t, psi, lpsi);
                 for (size_t i = first; i < n - 1; ++i) {
                     if (psi[i] > thr && u[i] - psi[i] < 1e-8 && lpsi[i] < 0.0) {
                         lower[i - 1] = 0.0; diag[i] = 1.0; upper[i] = 0.0; rhs_bc[i] = psi[i];
@@ -781,36 +1004,64 @@ inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, cons
     for (size_t i = 0; i < n; ++i) u[i] = std::max(u[i], psi[i]);
     u_prev = u;
 
+    // Discrete dividends as temporal events at tau = T - t_div
+    // (american_option.cpp:83-144; pde_solver.hpp:200-273): after the step
+    // that reaches tau, u(x) <- u(ln(e^x - D/K)) by the natural spline of u
+    // (the payoff value of S = 0 below the grid or when e^x <= D/K), then the
+    // boundary values and the obstacle are re-applied.
+    struct Event { double tau, d; };
+    std::vector<Event> events;
+    for (const auto& dv : filter_and_merge_dividends(p.dividends, p.maturity))
+        events.push_back({p.maturity - dv.calendar_time, dv.amount / p.strike});
+    std::sort(events.begin(), events.end(), [](const Event& a, const Event& b) { return a.tau < b.tau; });
+    size_t next_event = 0;
+    const double fallback = put ? 1.0 : 0.0;
+    auto process_events = [&](double t_old, double t_new) {
+        while (next_event < events.size()) {
+            const Event& e = events[next_event];
+            if (e.tau <= t_old) { ++next_event; continue; }
+            if (e.tau > t_new) break;
+            dividend_shift(x, u, e.d, fallback);
+            apply_bc(u, e.tau);
+            for (size_t i = 0; i < n; ++i) u[i] = std::max(u[i], psi[i]);
+            ++next_event;
+        }
+    };
+
     const double gamma = opts.trbdf2.gamma;
     double t = td.time_point(0);
     record(0, u);
     size_t step = 0;
     if (opts.trbdf2.rannacher_startup && td.n_steps > 0) {
         // solve_rannacher_startup (pde_solver.hpp:413-447)
-        const double dt = td.dt;
+        const double dt = td.dt_at(0);
         const double t_next = t + dt;
         const double half_dt = 0.5 * dt;
         const double t_mid = t + half_dt;
+        const double t_old = t;
         u_prev = u;
         rhs = u_prev; u = u_prev;
         tape_kind = 0;
         tape_un = nullptr;
         if (!stage(t_mid, half_dt, u, rhs)) return kLinearSolveFailure;
+        process_events(t_old, t_mid);
         u_prev = u;
         rhs = u_prev; u = u_prev;
         if (!stage(t_next, half_dt, u, rhs)) return kLinearSolveFailure;
+        process_events(t_mid, t_next);
         t = td.time_point(1);
         record(1, u);
         step = 1;
     }
     for (; step < td.n_steps; ++step) {
-        const double dt = td.dt;
+        const double t_old = t;
+        const double dt = td.dt_at(step);
         const double t_next = td.time_point(step + 1);
         u_prev = u;
         // stage 1 (pde_solver.hpp:342-362)
         const double t_stage1 = t + gamma * dt;
         const double w1 = gamma * dt / 2.0;
-        apply_L(u_prev, lu);
+        apply_L(t, u_prev, lu);
         for (size_t i = 0; i < n; ++i) rhs[i] = std::fma(w1, lu[i], u_prev[i]);
         u = u_prev;
         tape_kind = 1;
@@ -827,6 +1078,7 @@ inline int32_t solve_american(const PricingParams& p, const GridSpec& spec, cons
         tape_kind = 2;
         tape_un = nullptr;
         if (!stage(t_next, w2, u, rhs)) return kLinearSolveFailure;
+        process_events(t_old, t_next);
         t = t_next;
         record(step + 1, u);
     }
@@ -929,6 +1181,7 @@ struct BatchSolver {
     bool is_normalized_eligible(const std::vector<PricingParams>& ps, bool shared) const {
         if (!shared || ps.empty()) return false;
         for (size_t i = 1; i < ps.size(); ++i) if (ps[i].type != ps[0].type) return false;
+        for (const auto& p : ps) if (!p.dividends.empty()) return false;
         for (const auto& p : ps) if (p.spot <= 0.0 || p.strike <= 0.0) return false;
         auto [spec, td] = estimate_pde_grid(ps[0], accuracy);
         (void)td;
@@ -966,7 +1219,13 @@ struct BatchSolver {
             else if (resolved_custom) { spec = resolved_custom->first; nt = resolved_custom->second; }
             else { auto g = estimate_pde_grid(ps[i], accuracy); spec = g.first; nt = g.second.n_steps; }
             if (!validate_pricing_params(ps[i])) { res[i].status = kValidation; continue; }
-            const auto td = TimeDomain::from_n_steps(0.0, ps[i].maturity, nt);
+            // per-contract dividend dates as mandatory points (:448-470, resolved
+            // by american_option.cpp:66-71)
+            const std::vector<double> mand = mandatory_taus(ps[i]);
+            const auto td = mand.empty()
+                ? TimeDomain::from_n_steps(0.0, ps[i].maturity, nt)
+                : TimeDomain::with_mandatory_points(0.0, ps[i].maturity,
+                                                    ps[i].maturity / static_cast<double>(nt), mand);
             res[i].status = solve_american(ps[i], spec, td, res[i].sol, opts);
             if (res[i].status == kOk) res[i].greeks = compute_greeks(res[i].sol, ps[i].spot, ps[i].strike);
         }
@@ -983,14 +1242,17 @@ struct BatchSolver {
             const auto& p = ps[i];
             bool found = false;
             for (auto& g : groups) {
-                if (std::abs(g.p.volatility - p.volatility) < TOL && std::abs(g.p.rate - p.rate) < TOL &&
+                bool rate_match = false;
+                if (g.p.curve.has_value() == p.curve.has_value())
+                    rate_match = p.curve ? (*g.p.curve == *p.curve) : std::abs(g.p.rate - p.rate) < TOL;
+                if (std::abs(g.p.volatility - p.volatility) < TOL && rate_match &&
                     std::abs(g.p.dividend_yield - p.dividend_yield) < TOL &&
                     std::abs(g.p.maturity - p.maturity) < TOL && g.p.type == p.type) {
                     g.idx.push_back(i); found = true; break;
                 }
             }
             if (!found) {
-                PricingParams np{1.0, 1.0, p.maturity, p.rate, p.dividend_yield, p.volatility, p.type};
+                PricingParams np{1.0, 1.0, p.maturity, p.rate, p.dividend_yield, p.volatility, p.type, p.curve, {}};
                 groups.push_back({np, {i}});
             }
         }

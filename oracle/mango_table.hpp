@@ -441,7 +441,7 @@ inline bool build_price_table(const std::array<std::vector<double>, 4>& axes, do
     // make_batch (:200-230): sigma outer, rate inner, S = K = K_ref, T = tau_max
     std::vector<PricingParams> batch;
     for (double s : sig)
-        for (double r : rate) batch.push_back(PricingParams{K_ref, K_ref, tau.back(), r, q, s, type});
+        for (double r : rate) batch.push_back(PricingParams{K_ref, K_ref, tau.back(), r, q, s, type, std::nullopt, {}});
     BatchSolver bs;
     bs.n_threads = n_threads;
     bs.opts.snapshot_times = tau;

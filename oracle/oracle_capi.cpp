@@ -16,8 +16,11 @@ using namespace oracle;
 
 namespace {
 
-PricingParams to_params(const MangoPricingParams& p) {
-    PricingParams o;
+// build_pricing_params (src/ffi/mango_c_api.cpp:42-91): a tenor list becomes
+// a YieldCurve (false on an invalid curve or a null list), dividends are
+// copied as given.
+bool to_params_ex(const MangoPricingParams& p, PricingParams& o) {
+    o = PricingParams{};
     o.spot = p.spot;
     o.strike = p.strike;
     o.maturity = p.maturity;
@@ -25,7 +28,33 @@ PricingParams to_params(const MangoPricingParams& p) {
     o.dividend_yield = p.dividend_yield;
     o.volatility = p.volatility;
     o.type = p.option_type == MANGO_PUT ? OptionType::PUT : OptionType::CALL;
+    bool ok = true;
+    if (p.n_tenor_points > 0) {
+        if (!p.tenor_points) return false;
+        std::vector<TenorPoint> pts;
+        for (uint64_t i = 0; i < p.n_tenor_points; ++i) pts.push_back({p.tenor_points[i].tenor, p.tenor_points[i].log_discount});
+        o.curve = YieldCurve::from_points(std::move(pts));
+        ok = o.curve.has_value();
+    }
+    if (p.n_dividends > 0) {
+        if (!p.dividends) return false;
+        for (uint64_t i = 0; i < p.n_dividends; ++i) o.dividends.push_back({p.dividends[i].calendar_time, p.dividends[i].amount});
+    }
+    return ok;
+}
+
+PricingParams to_params(const MangoPricingParams& p) {
+    PricingParams o;
+    (void)to_params_ex(p, o);
     return o;
+}
+
+// PDEGridConfig{spec, n_time, dividend dates} resolved per contract
+// (american_option_batch.cpp:448-470; american_option.cpp:66-71).
+TimeDomain contract_time(const PricingParams& p, size_t n_time) {
+    const std::vector<double> mand = mandatory_taus(p);
+    if (mand.empty()) return TimeDomain::from_n_steps(0.0, p.maturity, n_time);
+    return TimeDomain::with_mandatory_points(0.0, p.maturity, p.maturity / static_cast<double>(n_time), mand);
 }
 
 GridAccuracyParams to_acc(const MangoGridAccuracy* a) {
@@ -161,8 +190,13 @@ uint64_t oracle_price_batch(const MangoPricingParams* p, uint64_t n, int32_t use
                             MangoAmericanGreeks* out, int32_t* status, int32_t n_threads,
                             uint64_t* pde_solves) {
     std::vector<PricingParams> ps;
+    std::vector<char> bad(n, 0);
     ps.reserve(n);
-    for (uint64_t i = 0; i < n; ++i) ps.push_back(to_params(p[i]));
+    for (uint64_t i = 0; i < n; ++i) {
+        PricingParams o;
+        bad[i] = to_params_ex(p[i], o) ? 0 : 1;
+        ps.push_back(o);
+    }
     BatchSolver bs;
     bs.use_normalized = use_normalized != 0;
     bs.n_threads = n_threads > 0 ? n_threads : 1;
@@ -175,7 +209,7 @@ uint64_t oracle_price_batch(const MangoPricingParams* p, uint64_t n, int32_t use
         for (uint64_t i = 0; i < n; ++i) {
             if (!validate_pricing_params(ps[i])) { res[i].status = kValidation; continue; }
             auto g = estimate_pde_grid(ps[i], acc);
-            res[i].status = solve_american(ps[i], g.first, TimeDomain::from_n_steps(0.0, ps[i].maturity, grid->n_time), res[i].sol);
+            res[i].status = solve_american(ps[i], g.first, contract_time(ps[i], grid->n_time), res[i].sol);
             if (res[i].status == kOk) res[i].greeks = compute_greeks(res[i].sol, ps[i].spot, ps[i].strike);
         }
         bs.pde_solves = n;
@@ -188,7 +222,7 @@ uint64_t oracle_price_batch(const MangoPricingParams* p, uint64_t n, int32_t use
             if (!validate_pricing_params(ps[i])) { res[i].status = kValidation; continue; }
             auto g = estimate_pde_grid(ps[i], acc);
             const size_t nt = std::max<size_t>(g.second.n_steps, grid->min_time_steps);
-            res[i].status = solve_american(ps[i], g.first, TimeDomain::from_n_steps(0.0, ps[i].maturity, nt), res[i].sol);
+            res[i].status = solve_american(ps[i], g.first, contract_time(ps[i], nt), res[i].sol);
             if (res[i].status == kOk) res[i].greeks = compute_greeks(res[i].sol, ps[i].spot, ps[i].strike);
         }
         bs.pde_solves = n;
@@ -198,6 +232,8 @@ uint64_t oracle_price_batch(const MangoPricingParams* p, uint64_t n, int32_t use
         if (grid && grid->mode == MANGO_GRID_MODE_EXPLICIT) custom = PDEGridConfig{to_spec(grid->spec), grid->n_time};
         res = bs.solve_batch(ps, use_shared_grid != 0, custom);
     }
+    for (uint64_t i = 0; i < n; ++i)
+        if (bad[i]) res[i].status = kValidation;
     uint64_t failed = 0;
     for (uint64_t i = 0; i < n; ++i) {
         status[i] = map_status(res[i].status);
@@ -455,6 +491,85 @@ int32_t oracle_operator(const MangoPricingParams* p, const double* x, uint64_t n
         ju[i] = a / (dr * da) + b * dl / (dr * (dl + dr));
     }
     return 0;
+}
+
+// mango_price_american (src/ffi/mango_c_api.cpp:309-358): the estimated grid
+// and time domain of AmericanOptionSolver::create(params) with default
+// accuracy. *n_steps and *dt receive the march's step count and average dt.
+int32_t oracle_price_single(const MangoPricingParams* p, MangoAmericanGreeks* greeks, uint64_t* n_steps, double* dt) {
+    PricingParams pp;
+    if (!to_params_ex(*p, pp)) return MANGO_ERR_VALIDATION;
+    if (!validate_pricing_params(pp)) return MANGO_ERR_VALIDATION;
+    auto [g, td] = estimate_pde_grid(pp);
+    Solution s;
+    const int32_t st = solve_american(pp, g, td, s);
+    if (st != kOk) return map_status(st);
+    const auto gk = compute_greeks(s, pp.spot, pp.strike);
+    *greeks = {gk.value, gk.delta, gk.gamma, gk.theta};
+    if (n_steps) *n_steps = td.n_steps;
+    if (dt) *dt = td.dt;
+    return MANGO_OK;
+}
+
+// One solve on an explicit grid and an explicit time domain
+// (with_mandatory_points(0, T, dt_max, mandatory) when n_mand > 0, else
+// from_n_steps(0, T, n_time)); x/u/u_prev may be null.
+int32_t oracle_solve_ex(const MangoPricingParams* p, const MangoGridSpec* spec, uint64_t n_time, double dt_max,
+                        const double* mand, uint64_t n_mand, double* x, double* u, double* u_prev,
+                        MangoAmericanGreeks* greeks) {
+    PricingParams pp;
+    if (!to_params_ex(*p, pp)) return MANGO_ERR_VALIDATION;
+    const TimeDomain td = n_mand > 0
+        ? TimeDomain::with_mandatory_points(0.0, pp.maturity, dt_max, std::vector<double>(mand, mand + n_mand))
+        : TimeDomain::from_n_steps(0.0, pp.maturity, n_time);
+    Solution s;
+    const int32_t st = solve_american(pp, to_spec(*spec), td, s);
+    if (st != kOk) return map_status(st);
+    if (x) std::copy(s.x.begin(), s.x.end(), x);
+    if (u) std::copy(s.u.begin(), s.u.end(), u);
+    if (u_prev) std::copy(s.u_prev.begin(), s.u_prev.end(), u_prev);
+    if (greeks) {
+        const auto g = compute_greeks(s, pp.spot, pp.strike);
+        *greeks = {g.value, g.delta, g.gamma, g.theta};
+    }
+    return MANGO_OK;
+}
+
+// YieldCurve::from_points, then rate / log_discount / discount at t[i].
+// Returns 0, or 1 with the reason in why (cap bytes) on an invalid curve.
+int32_t oracle_curve_eval(const MangoTenorPoint* pts, uint64_t n, const double* t, uint64_t nt, double* rate,
+                          double* log_discount, double* discount, char* why, uint64_t cap) {
+    std::vector<TenorPoint> v;
+    for (uint64_t i = 0; i < n; ++i) v.push_back({pts[i].tenor, pts[i].log_discount});
+    std::string msg;
+    auto c = YieldCurve::from_points(std::move(v), &msg);
+    if (!c) {
+        if (why && cap > 0) { std::strncpy(why, msg.c_str(), cap - 1); why[cap - 1] = 0; }
+        return 1;
+    }
+    for (uint64_t i = 0; i < nt; ++i) {
+        rate[i] = c->rate(t[i]);
+        log_discount[i] = c->log_discount(t[i]);
+        discount[i] = c->discount(t[i]);
+    }
+    return 0;
+}
+
+// TimeDomain::with_mandatory_points(t0, t1, dt, mandatory): writes the points
+// (n_steps + 1 of them, at most cap) and returns n_steps; *dt_avg = dt().
+uint64_t oracle_time_points(double t0, double t1, double dt, const double* mand, uint64_t n_mand, double* out,
+                            uint64_t cap, double* dt_avg) {
+    const auto td = TimeDomain::with_mandatory_points(t0, t1, dt, std::vector<double>(mand, mand + n_mand));
+    for (uint64_t i = 0; i <= td.n_steps && i < cap; ++i) out[i] = td.time_point(i);
+    if (dt_avg) *dt_avg = td.dt;
+    return td.n_steps;
+}
+
+// The dividend event callback on (x, u) in place.
+void oracle_dividend_shift(const double* x, double* u, uint64_t n, double d, double fallback) {
+    std::vector<double> xv(x, x + n), uv(u, u + n);
+    dividend_shift(xv, uv, d, fallback);
+    std::copy(uv.begin(), uv.end(), u);
 }
 
 double oracle_bs_price(double S, double K, double tau, double sigma, double r, double q, int32_t type) {

@@ -13,8 +13,8 @@ import subprocess
 
 import numpy as np
 
-from mango_option._abi import (MangoAmericanGreeks, MangoGridAccuracy, MangoGridConfig, MangoGridSpec,
-                               MangoPricingParams, MangoSinhCluster)
+from mango_option._abi import (MangoAmericanGreeks, MangoDividend, MangoGridAccuracy, MangoGridConfig,
+                               MangoGridSpec, MangoPricingParams, MangoSinhCluster, MangoTenorPoint)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
@@ -74,6 +74,14 @@ def load() -> C.CDLL:
         "oracle_thomas_projected": (C.c_int32, [_dp, _dp, _dp, _dp, _dp, _dp, C.c_uint64]),
         "oracle_spline_eval": (C.c_int32, [_dp, _dp, C.c_uint64, _dp, C.c_uint64, _dp, _dp]),
         "oracle_operator": (C.c_int32, [P(MangoPricingParams), _dp, C.c_uint64, _dp, _dp, _dp, _dp, _dp]),
+        "oracle_price_single": (C.c_int32, [P(MangoPricingParams), P(MangoAmericanGreeks), P(C.c_uint64), _dp]),
+        "oracle_solve_ex": (C.c_int32, [P(MangoPricingParams), P(MangoGridSpec), C.c_uint64, C.c_double, _dp,
+                                        C.c_uint64, _dp, _dp, _dp, P(MangoAmericanGreeks)]),
+        "oracle_curve_eval": (C.c_int32, [P(MangoTenorPoint), C.c_uint64, _dp, C.c_uint64, _dp, _dp, _dp,
+                                          C.c_char_p, C.c_uint64]),
+        "oracle_time_points": (C.c_uint64, [C.c_double, C.c_double, C.c_double, _dp, C.c_uint64, _dp, C.c_uint64,
+                                            _dp]),
+        "oracle_dividend_shift": (None, [_dp, _dp, C.c_uint64, C.c_double, C.c_double]),
         "oracle_bs_price": (C.c_double, [C.c_double] * 6 + [C.c_int32]),
         "oracle_bs_vega": (C.c_double, [C.c_double] * 6),
     }
@@ -334,3 +342,74 @@ def bspline_eval4_d2(axes, coeffs, points, axis):
     out = np.empty(pts.shape[0])
     load().oracle_bspline_eval4_d2(*_axes_args(axes), _p(c), _p(pts), pts.shape[0], int(axis), _p(out))
     return out
+
+
+class Params:
+    """One MangoPricingParams with an optional yield curve (list of (tenor,
+    log_discount)) and discrete dividends (list of (calendar_time, amount));
+    holds the arrays the struct points into."""
+
+    def __init__(self, spot, strike, maturity, vol, rate, q, put=True, curve=None, dividends=None):
+        self.p = one_param(spot, strike, maturity, vol, rate, q, put)
+        self._curve = self._divs = None
+        if curve:
+            self._curve = (MangoTenorPoint * len(curve))(*[MangoTenorPoint(float(a), float(b)) for a, b in curve])
+            self.p.tenor_points = C.cast(self._curve, C.POINTER(MangoTenorPoint))
+            self.p.n_tenor_points = len(curve)
+        if dividends:
+            self._divs = (MangoDividend * len(dividends))(*[MangoDividend(float(a), float(b)) for a, b in dividends])
+            self.p.dividends = C.cast(self._divs, C.POINTER(MangoDividend))
+            self.p.n_dividends = len(dividends)
+
+
+def params_array(items):
+    """MangoPricingParams[n] from Params objects (which must stay alive)."""
+    arr = (MangoPricingParams * max(1, len(items)))()
+    for i, it in enumerate(items):
+        arr[i] = it.p
+    return arr, len(items)
+
+
+def price_single(pp: Params):
+    """mango_price_american restated: (status, greeks[4], n_steps, dt)."""
+    g = MangoAmericanGreeks()
+    ns = C.c_uint64()
+    dt = C.c_double()
+    st = load().oracle_price_single(C.byref(pp.p), C.byref(g), C.byref(ns), C.byref(dt))
+    return st, np.array([g.value, g.delta, g.gamma, g.theta]), ns.value, dt.value
+
+
+def solve_ex(pp: Params, spec: MangoGridSpec, n_time: int, dt_max: float = 0.0, mandatory=()):
+    mand = _a(list(mandatory) if len(mandatory) else [0.0])
+    n = spec.n_points
+    x = np.zeros(n); u = np.zeros(n); up = np.zeros(n)
+    g = MangoAmericanGreeks()
+    st = load().oracle_solve_ex(C.byref(pp.p), C.byref(spec), n_time, dt_max, _p(mand), len(mandatory), _p(x),
+                                _p(u), _p(up), C.byref(g))
+    return st, {"x": x, "u": u, "u_prev": up, "greeks": np.array([g.value, g.delta, g.gamma, g.theta])}
+
+
+def curve_eval(points, t):
+    """YieldCurve::from_points(points) at times t: (ok, rate, log_discount, discount, why)."""
+    arr = (MangoTenorPoint * max(1, len(points)))(*[MangoTenorPoint(float(a), float(b)) for a, b in points])
+    t = _a(t)
+    r = np.zeros(len(t)); ld = np.zeros(len(t)); d = np.zeros(len(t))
+    why = C.create_string_buffer(128)
+    st = load().oracle_curve_eval(arr, len(points), _p(t), len(t), _p(r), _p(ld), _p(d), why, 128)
+    return st == 0, r, ld, d, why.value.decode()
+
+
+def time_points(t0, t1, dt, mandatory):
+    """TimeDomain::with_mandatory_points: (points, dt())."""
+    mand = _a(list(mandatory) if len(mandatory) else [0.0])
+    cap = int(np.ceil((t1 - t0) / dt)) + len(mandatory) + 8
+    out = np.zeros(cap)
+    dta = C.c_double()
+    n = load().oracle_time_points(t0, t1, dt, _p(mand), len(mandatory), _p(out), cap, C.byref(dta))
+    return out[:n + 1].copy(), dta.value
+
+
+def dividend_shift(x, u, d, fallback):
+    x = _a(x); u = _a(u).copy()
+    load().oracle_dividend_shift(_p(x), _p(u), len(x), d, fallback)
+    return u
