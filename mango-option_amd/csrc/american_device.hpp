@@ -529,11 +529,8 @@ struct OptionPde {
         group_sync();
         if (valid && lane == 0 && needs_monotonicity(job.grid)) enforce_monotonicity(xs, n, job.grid.x_min, job.grid.x_max);
         group_sync();
-        r = job.rate;
         q = job.dividend_yield;
         hss = 0.5 * job.volatility * job.volatility;
-        drift = r - q - hss;
-        lockable = 0;
         bool nonuni = false;
         double my_psi_max = -kBig, my_dx_min = kBig;
         expected_dx = valid ? (xs[n - 1] - xs[0]) / static_cast<double>(n - 1) : 1.0;
@@ -541,7 +538,6 @@ struct OptionPde {
         for (int k = 0; k < M; ++k) {
             const int i = base + k;
             psi[k] = -kBig;
-            jl[k] = jd[k] = ju[k] = 0.0;
             if (valid && i < n) {
                 psi[k] = payoff(xs[i]);
                 my_psi_max = smax(my_psi_max, psi[k]);
@@ -550,53 +546,65 @@ struct OptionPde {
                     my_dx_min = smin(my_dx_min, dx);
                     if (fabs(dx - expected_dx) > 100.0 * 2.220446049250313e-16) nonuni = true;
                 }
-                if (i >= 1 && i <= n - 2) {
-                    const double dl = xs[i] - xs[i - 1];
-                    const double dr = xs[i + 1] - xs[i];
-                    const double a = hss, b = r - q - hss, c = -r;
-                    const double dx_avg = (dl + dr) / 2.0;
-                    const double d2_im1 = a / (dl * dx_avg);
-                    const double d2_i = -a * (1.0 / dl + 1.0 / dr) / dx_avg;
-                    const double d2_ip1 = a / (dr * dx_avg);
-                    const double d1_denom = dl + dr;
-                    const double d1_im1 = -b * dr / (dl * d1_denom);
-                    const double d1_i = b * (dr - dl) / (dl * dr);
-                    const double d1_ip1 = b * dl / (dr * d1_denom);
-                    jl[k] = d2_im1 + d1_im1;
-                    jd[k] = d2_i + d1_i + c;
-                    ju[k] = d2_ip1 + d1_ip1;
-                }
             }
         }
         uniform = (__ballot(nonuni) & gmask) == 0;
         psi_max = group_max<K>(my_psi_max);
         dx_min = group_min<K>(my_dx_min);
-        {
-            const double thr = kDeepItmFraction * psi_max;
-#pragma unroll
-            for (int k = 0; k < M; ++k) {
-                const int i = base + k;
-                if (valid && psi_max > 0.0 && i >= 1 && i <= n - 2 && psi[k] > thr) {
-                    double pv[3] = {payoff(xs[i - 1]), psi[k], payoff(xs[i + 1])};
-                    double d1, d2;
-                    stencil_at(xs + (i - 1), pv, 3, false, 1, d1, d2);
-                    if (uniform) {
-                        const double dx = expected_dx;
-                        const double dx_inv = 1.0 / dx;
-                        const double dx2_inv = dx_inv * dx_inv;
-                        d2 = fma(pv[2] + pv[0], dx2_inv, -2.0 * pv[1] * dx2_inv);
-                        d1 = (pv[2] - pv[0]) * (dx_inv * 0.5);
-                    }
-                    if (hss * d2 + drift * d1 - r * pv[1] < 0.0) lockable |= (1u << k);
-                }
-            }
-        }
+        set_rate(job.rate, xs);
         const double x_first = valid ? xs[0] : 0.0;
         x_last = valid ? xs[n - 1] : 0.0;
         g_left = put ? smax(1.0 - exp(x_first), 0.0) : 0.0;
         ex_last = put ? 0.0 : exp(x_last);
         e_dt = e_g = 0.0;
         k_right = (n - 1 >= base && n - 1 < base + M) ? n - 1 - base : -1;
+    }
+
+    // Jacobian bands of L at rate r (spatial_operator.hpp:123-164) and the
+    // deep-ITM lock eligibility L(psi) < 0 at that rate (pde_solver.hpp:631-680).
+    __device__ __forceinline__ void set_rate(double r_new, const double* __restrict__ xs) {
+        r = r_new;
+        drift = r - q - hss;
+        lockable = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            jl[k] = jd[k] = ju[k] = 0.0;
+            if (valid && i >= 1 && i <= n - 2) {
+                const double dl = xs[i] - xs[i - 1];
+                const double dr = xs[i + 1] - xs[i];
+                const double a = hss, b = r - q - hss, c = -r;
+                const double dx_avg = (dl + dr) / 2.0;
+                const double d2_im1 = a / (dl * dx_avg);
+                const double d2_i = -a * (1.0 / dl + 1.0 / dr) / dx_avg;
+                const double d2_ip1 = a / (dr * dx_avg);
+                const double d1_denom = dl + dr;
+                const double d1_im1 = -b * dr / (dl * d1_denom);
+                const double d1_i = b * (dr - dl) / (dl * dr);
+                const double d1_ip1 = b * dl / (dr * d1_denom);
+                jl[k] = d2_im1 + d1_im1;
+                jd[k] = d2_i + d1_i + c;
+                ju[k] = d2_ip1 + d1_ip1;
+            }
+        }
+        const double thr = kDeepItmFraction * psi_max;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            if (valid && psi_max > 0.0 && i >= 1 && i <= n - 2 && psi[k] > thr) {
+                double pv[3] = {payoff(xs[i - 1]), psi[k], payoff(xs[i + 1])};
+                double d1, d2;
+                stencil_at(xs + (i - 1), pv, 3, false, 1, d1, d2);
+                if (uniform) {
+                    const double dx = expected_dx;
+                    const double dx_inv = 1.0 / dx;
+                    const double dx2_inv = dx_inv * dx_inv;
+                    d2 = fma(pv[2] + pv[0], dx2_inv, -2.0 * pv[1] * dx2_inv);
+                    d1 = (pv[2] - pv[0]) * (dx_inv * 0.5);
+                }
+                if (hss * d2 + drift * d1 - r * pv[1] < 0.0) lockable |= (1u << k);
+            }
+        }
     }
 
     // Value held by lane + 1 (the node block just below) / lane - 1 (just

@@ -1,0 +1,354 @@
+// SPDX-License-Identifier: MIT
+//
+// gfx950 event kernel: the American solve with a yield curve (the rate of
+// each stage from the curve's forward rates, the call boundary from its
+// forward discount), a piecewise-uniform time domain with mandatory points,
+// and discrete cash dividends applied as temporal events.
+//
+// Reference behaviour:
+//   march, Rannacher start, events in (t_old, t_new]  src/pde/internal/pde_solver.hpp:113-178, 235-273, 413-447
+//   rate r(tau) = curve.rate(max(0, T - tau))         src/option/option_spec.hpp:50-64
+//   call boundary e^x - D(T)/D(T - tau)                src/option/american_option.cpp:276-288; option_spec.hpp:75-91
+//   bands and lock test at the stage's rate           src/pde/internal/spatial_operator.hpp:123-164; pde_solver.hpp:590-680
+//   dividend shift u(x) <- s_u(ln(e^x - D/K))          src/option/american_option.cpp:79-144
+//   curve forward rate / log-linear discount           src/option/yield_curve.hpp:104-178
+//
+// One job per wavefront (64 lanes x M nodes, the solve kernel's layout). The
+// stage factorizations are cached per stage kind while the weight and the
+// rate are unchanged (piecewise-constant forward rates change them only at
+// tenors, dt only at mandatory points). A dividend writes the state to LDS,
+// lane 0 fits the natural spline, and every lane evaluates its nodes' shifted
+// points.
+#include "american_device.hpp"
+#include "mango_events.hpp"
+
+namespace mango_amd {
+namespace {
+
+constexpr int kEventLanes = 64;
+
+// One job's record in the pool (layout documented in mango_events.hpp).
+struct EventRecord {
+    int n_curve, n_seg, n_ev;
+    int64_t n_steps;
+    double dt_avg;
+    const double* tenor;
+    const double* logd;
+    const double* seg_start;
+    const double* sub_dt;
+    const double* n_sub;
+    const double* ev_tau;
+    const double* ev_d;
+
+    __device__ __forceinline__ void load(const double* __restrict__ pool, int64_t ext) {
+        const double* r = pool + (ext - 1);
+        n_curve = static_cast<int>(r[0]);
+        n_seg = static_cast<int>(r[1]);
+        n_ev = static_cast<int>(r[2]);
+        n_steps = static_cast<int64_t>(r[3]);
+        dt_avg = r[4];
+        tenor = r + 5;
+        logd = tenor + n_curve;
+        seg_start = logd + n_curve;
+        sub_dt = seg_start + n_seg;
+        n_sub = sub_dt + n_seg;
+        ev_tau = n_sub + n_seg;
+        ev_d = ev_tau + n_ev;
+    }
+    // index of the first tenor > t (std::upper_bound)
+    __device__ int upper(double t) const {
+        int lo = 0, hi = n_curve;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (t < tenor[mid]) hi = mid; else lo = mid + 1;
+        }
+        return lo;
+    }
+    __device__ double rate_between(int i) const {
+        if (i + 1 >= n_curve) return 0.0;
+        const double dt = tenor[i + 1] - tenor[i];
+        if (dt <= 0.0) return 0.0;
+        return -(logd[i + 1] - logd[i]) / dt;
+    }
+    __device__ double curve_rate(double t) const {
+        if (n_curve < 2) return 0.0;
+        if (t <= 0.0) return rate_between(0);
+        const int ub = upper(t);
+        if (ub == 0) return rate_between(0);
+        if (ub == n_curve) return rate_between(n_curve - 2);
+        return rate_between(ub - 1);
+    }
+    __device__ double log_discount(double t) const {
+        if (n_curve < 2) return 0.0;
+        if (t <= 0.0) return 0.0;
+        const int ub = upper(t);
+        if (ub == 0) return 0.0;
+        if (ub == n_curve) {
+            const double r = -(logd[n_curve - 1] - logd[n_curve - 2]) / (tenor[n_curve - 1] - tenor[n_curve - 2]);
+            return logd[n_curve - 1] - r * (t - tenor[n_curve - 1]);
+        }
+        const double a = (t - tenor[ub - 1]) / (tenor[ub] - tenor[ub - 1]);
+        return logd[ub - 1] + a * (logd[ub] - logd[ub - 1]);
+    }
+    __device__ double discount(double t) const { return exp(log_discount(t)); }
+};
+
+template <int M>
+__device__ void solve_event_job(const PdeJob& job, const double* __restrict__ pool, const int lane,
+                                double* __restrict__ xs, double* __restrict__ ys, double* __restrict__ cps,
+                                double* __restrict__ cs, Greeks* __restrict__ greeks_out,
+                                int32_t* __restrict__ status_out, const StateOut& state) {
+    constexpr int K = kEventLanes;
+    OptionPde<K, M> P;
+    P.init(job, lane, xs);
+    EventRecord rec;
+    rec.load(pool, job.ext);
+    const double T = job.maturity;
+    const bool put = P.put;
+    const int n = P.n;
+    const int base = P.base;
+
+    auto rate_at = [&](double tau) { return rec.n_curve ? rec.curve_rate(smax(0.0, T - tau)) : job.rate; };
+    auto g_right = [&](double tau) -> double {
+        if (put) return 0.0;
+        if (!rec.n_curve) return P.ex_last - exp(-job.rate * tau);
+        const double s = smax(0.0, T - tau);
+        return P.ex_last - rec.discount(T) / rec.discount(s);
+    };
+
+    // factorization caches: valid while (weight, band version) are unchanged
+    StageCache<M> c0, c1;
+    c0.invalidate();
+    c1.invalidate();
+    double c0_w = NAN, c1_w = NAN;
+    int c0_v = -1, c1_v = -1, ver = 0;
+    auto ensure_rate = [&](double r) {
+        if (r != P.r) { P.set_rate(r, xs); ++ver; }
+    };
+    auto use_cache = [&](StageCache<M>& c, double& cw, int& cv, double w) {
+        if (w != cw || cv != ver) { c.invalidate(); cw = w; cv = ver; }
+    };
+
+    // discrete dividend at time-to-expiry te, then boundary values and obstacle
+    const double fallback = put ? 1.0 : 0.0;
+    auto apply_dividend = [&](double d, double te, double (&u)[M]) {
+        if (d > 0.0) {
+#pragma unroll
+            for (int k = 0; k < M; ++k) { const int i = base + k; if (i < n) ys[i] = u[k]; }
+            group_sync();
+            int fit = 0;
+            if (lane == 0) fit = spline_second(xs, ys, n, cps, cs) ? 1 : 0;
+            fit = __shfl(fit, 0, K);
+            group_sync();
+            if (fit) {
+                const double x_lo = xs[0], x_hi = xs[n - 1];
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    const int i = base + k;
+                    if (i < n) {
+                        const double s_adj = exp(xs[i]) - d;
+                        double v = fallback;
+                        if (s_adj > 1e-10) {
+                            const double xsh = log(s_adj);
+                            if (!(xsh < x_lo)) v = spline_eval(xs, ys, cs, n, smin(xsh, x_hi));
+                        }
+                        u[k] = v;
+                    }
+                }
+            }
+            group_sync();
+        }
+        const double gr = g_right(te);
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            if (i == 0) u[k] = P.g_left;
+            if (i == n - 1) u[k] = gr;
+            if (i < n) u[k] = smax(u[k], P.psi[k]);
+        }
+    };
+    int ev = 0;
+    auto process_events = [&](double t_old, double t_new, double (&u)[M]) {
+        while (ev < rec.n_ev) {
+            const double te = rec.ev_tau[ev];
+            if (te <= t_old) { ++ev; continue; }
+            if (te > t_new) break;
+            apply_dividend(rec.ev_d[ev], te, u);
+            ++ev;
+        }
+    };
+
+    // time points: uniform t_i = i dt, or seg_start + j sub_dt per segment
+    int seg = 0;
+    int64_t sub = 0;
+    auto next_point = [&]() -> double {
+        ++sub;
+        if (rec.n_seg == 0) return static_cast<double>(sub) * rec.dt_avg;
+        if (sub > static_cast<int64_t>(rec.n_sub[seg])) { ++seg; sub = 1; }
+        return rec.seg_start[seg] + static_cast<double>(sub) * rec.sub_dt[seg];
+    };
+
+    double u[M], up[M];
+    P.initial(u);
+#pragma unroll
+    for (int k = 0; k < M; ++k) up[k] = u[k];
+    bool ok = P.valid && rec.n_steps > 0;
+    const TrConstants tr;
+    const int64_t n_steps = rec.n_steps;
+    double t = 0.0;
+    int64_t step = 0;
+    if (ok) {
+        // Rannacher start: two implicit-Euler half steps, events after each
+        const double p1 = next_point();
+        const double dt = (rec.n_seg == 0) ? rec.dt_avg : p1 - 0.0;
+        const double t_next = t + dt;
+        const double half_dt = 0.5 * dt;
+        const double t_mid = t + half_dt;
+#pragma unroll
+        for (int k = 0; k < M; ++k) up[k] = u[k];
+        ensure_rate(rate_at(t_mid));
+        use_cache(c0, c0_w, c0_v, half_dt);
+        ok = P.stage(g_right(t_mid), half_dt, up, up, c0, u);
+        if (ok) {
+            process_events(t, t_mid, u);
+#pragma unroll
+            for (int k = 0; k < M; ++k) up[k] = u[k];
+            ensure_rate(rate_at(t_next));
+            use_cache(c0, c0_w, c0_v, half_dt);
+            ok = P.stage(g_right(t_next), half_dt, up, up, c0, u);
+        }
+        if (ok) process_events(t_mid, t_next, u);
+        t = p1;
+        step = 1;
+    }
+    for (; ok && step < n_steps; ++step) {
+        const double t_old = t;
+        const double t_next = next_point();
+        const double dt = (rec.n_seg == 0) ? rec.dt_avg : t_next - t;
+#pragma unroll
+        for (int k = 0; k < M; ++k) up[k] = u[k];
+        // stage 1: rhs = u^n + w1 L(t_n) u^n, solved at t_n + gamma dt
+        const double t_s1 = t + tr.gamma * dt;
+        const double w1 = tr.w1(dt);
+        double rhs[M], mid[M];
+        ensure_rate(rate_at(t));
+        P.stage1_rhs(w1, up, rhs);
+        ensure_rate(rate_at(t_s1));
+        use_cache(c0, c0_w, c0_v, w1);
+        ok = P.stage(g_right(t_s1), w1, rhs, up, c0, u);
+        if (!ok) break;
+        // stage 2: rhs = alpha u* + beta u^n, solved at t_{n+1}
+#pragma unroll
+        for (int k = 0; k < M; ++k) { mid[k] = u[k]; rhs[k] = fma(tr.alpha, u[k], tr.beta * up[k]); }
+        const double w2 = tr.w2(dt);
+        ensure_rate(rate_at(t_next));
+        use_cache(c1, c1_w, c1_v, w2);
+        ok = P.stage(g_right(t_next), w2, rhs, mid, c1, u);
+        if (!ok) break;
+        process_events(t_old, t_next, u);
+        t = t_next;
+    }
+
+    // ---- epilogue: natural splines and Greeks on one lane (dt = T / n_steps) ----
+    const double dt_avg = rec.dt_avg;
+    const bool valid = P.valid;
+    const bool write_state = (job.flags & kJobWriteState) && state.u != nullptr;
+    const int64_t row = static_cast<int64_t>(job.out_index) * state.stride;
+    if (write_state && ok) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const int i = base + k;
+            if (i < n) {
+                state.x[row + i] = xs[i];
+                state.u[row + i] = u[k];
+                state.u_prev[row + i] = up[k];
+            }
+        }
+        if (lane == 0) { state.n_space[job.out_index] = n; state.dt[job.out_index] = dt_avg; }
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) { const int i = base + k; if (i < n && valid) ys[i] = u[k]; }
+    group_sync();
+    double v_cur = 0.0;
+    Greeks gk{NAN, NAN, NAN, NAN};
+    if (lane == 0 && ok) {
+        spline_second(xs, ys, n, cps, cs);
+        const double xsp = log(job.spot / job.strike);
+        v_cur = spline_eval(xs, ys, cs, n, xsp);
+        if (write_state) for (int i = 0; i < n; ++i) state.c_u[row + i] = cs[i];
+        if (job.flags & kJobGreeks) {
+            const Greeks g = greeks_at(xs, ys, cs, ys, cs, n, P.uniform, job.spot, job.strike, dt_avg);
+            gk.value = g.value; gk.delta = g.delta; gk.gamma = g.gamma;
+        }
+    }
+    group_sync();
+#pragma unroll
+    for (int k = 0; k < M; ++k) { const int i = base + k; if (i < n && valid) ys[i] = up[k]; }
+    group_sync();
+    if (lane == 0 && ok) {
+        spline_second(xs, ys, n, cps, cs);
+        if (write_state) for (int i = 0; i < n; ++i) state.c_up[row + i] = cs[i];
+        if (job.flags & kJobGreeks) {
+            const double xsp = log(job.spot / job.strike);
+            const double v_prev = spline_eval(xs, ys, cs, n, xsp);
+            gk.theta = (v_prev - v_cur) / dt_avg * job.strike;
+        }
+    }
+    if (lane == 0) {
+        int32_t st = kStatusOk;
+        if (!valid) st = params_valid(job) ? kStatusSolver : kStatusValidation;
+        else if (!ok) st = kStatusSolver;
+        if (st != kStatusOk) gk = Greeks{NAN, NAN, NAN, NAN};
+        if (job.flags & kJobGreeks) greeks_out[job.out_index] = gk;
+        status_out[job.out_index] = st;
+    }
+    group_sync();
+}
+
+template <int M>
+__global__ __launch_bounds__(kEventLanes) void american_events_kernel(const PdeJob* __restrict__ jobs, int n_jobs,
+                                                                      const double* __restrict__ pool,
+                                                                      Greeks* __restrict__ greeks_out,
+                                                                      int32_t* __restrict__ status_out,
+                                                                      StateOut state) {
+    constexpr int NMAX = kEventLanes * M;
+    __shared__ double s_x[NMAX];
+    __shared__ double s_y[NMAX];
+    __shared__ double s_cp[NMAX];
+    __shared__ double s_c[NMAX];
+    const int lane = threadIdx.x;
+    // one job per block; the loop condition is uniform over the wavefront
+    for (int j = blockIdx.x; j < n_jobs; j += gridDim.x)
+        solve_event_job<M>(jobs[j], pool, lane, s_x, s_y, s_cp, s_c, greeks_out, status_out, state);
+}
+
+}  // namespace
+
+int events_nodes_for(int64_t n_points) {
+    if (n_points <= kEventLanes * 2) return 2;
+    if (n_points <= kEventLanes * 4) return 4;
+    if (n_points <= kEventLanes * 7) return 7;
+    return 0;
+}
+
+int launch_american_events(int nodes_per_lane, const PdeJob* d_jobs, int n_jobs, const double* d_pool,
+                           Greeks* d_greeks, int32_t* d_status, const StateOut& state, hipStream_t stream) {
+    if (n_jobs <= 0) return 0;
+    const int blocks = n_jobs < 65535 ? n_jobs : 65535;
+    switch (nodes_per_lane) {
+        case 2:
+            american_events_kernel<2><<<blocks, kEventLanes, 0, stream>>>(d_jobs, n_jobs, d_pool, d_greeks, d_status, state);
+            break;
+        case 4:
+            american_events_kernel<4><<<blocks, kEventLanes, 0, stream>>>(d_jobs, n_jobs, d_pool, d_greeks, d_status, state);
+            break;
+        case 7:
+            american_events_kernel<7><<<blocks, kEventLanes, 0, stream>>>(d_jobs, n_jobs, d_pool, d_greeks, d_status, state);
+            break;
+        default:
+            return 1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+}  // namespace mango_amd

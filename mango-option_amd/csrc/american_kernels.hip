@@ -548,7 +548,7 @@ __global__ __launch_bounds__(64) void plan_chains_kernel(const DeviceOptionRecor
         PdeJob jb;
         jb.maturity = o.maturity; jb.volatility = o.volatility; jb.rate = o.rate;
         jb.dividend_yield = o.dividend_yield;
-        jb.spot = 1.0; jb.strike = 1.0; jb.c_t = plan.accuracy.c_t; jb.pad0 = 0.0;
+        jb.spot = 1.0; jb.strike = 1.0; jb.c_t = plan.accuracy.c_t; jb.ext = 0;
         jb.grid = g;
         jb.option_type = o.option_type;
         jb.n_time = plan.n_time;
@@ -667,7 +667,7 @@ __global__ __launch_bounds__(kBlock) void plan_binned_kernel(const DeviceOptionR
     const DeviceOptionRecord o = opts[i];
     PdeJob j;
     j.maturity = o.maturity; j.volatility = o.volatility; j.rate = o.rate; j.dividend_yield = o.dividend_yield;
-    j.spot = o.spot; j.strike = o.strike; j.c_t = plan.accuracy.c_t; j.pad0 = 0.0;
+    j.spot = o.spot; j.strike = o.strike; j.c_t = plan.accuracy.c_t; j.ext = 0;
     j.option_type = o.option_type;
     j.max_time_steps = plan.accuracy.max_time_steps;
     j.min_time_steps = 0;
@@ -750,7 +750,7 @@ __global__ __launch_bounds__(kBlock) void plan_jobs_kernel(const DeviceOptionRec
     j.spot = o.spot;
     j.strike = o.strike;
     j.c_t = plan.accuracy.c_t;
-    j.pad0 = 0.0;
+    j.ext = 0;
     j.option_type = o.option_type;
     j.max_time_steps = plan.accuracy.max_time_steps;
     j.min_time_steps = 0;

@@ -20,7 +20,7 @@ struct alignas(16) PdeJob {
     double spot;
     double strike;
     double c_t;               // auto step count: dt = c_t * dx_min (n_time == 0)
-    double pad0;
+    int64_t ext;              // 1 + offset of the job's curve/time/dividend record (event kernel), 0 = none
     GridDesc grid;
     int32_t option_type;      // 0 call, 1 put
     int32_t n_time;           // 0 => derive from the generated grid

@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "american_kernels.hpp"
+#include "mango_events.hpp"
 #include "mango_host.hpp"
 
 using namespace mango_amd;
@@ -98,9 +99,10 @@ void from_desc(const GridDesc& g, MangoGridSpec* s) {
     }
 }
 
-// validate_option_spec + validate_pricing_params; the device path supports a
-// constant rate and no discrete dividends.
-MangoStatus validate_params(const MangoPricingParams& p, const char** why) {
+// validate_option_spec + validate_pricing_params. With `events` the yield
+// curve and dividend checks of the reference apply (the event kernel serves
+// them); without, a curve or dividends make the option unsupported.
+MangoStatus validate_params(const MangoPricingParams& p, const char** why, bool events) {
     auto bad = [&](const char* m) { *why = m; return MANGO_ERR_VALIDATION; };
     if (p.option_type != MANGO_CALL && p.option_type != MANGO_PUT) return bad("invalid option_type");
     if (p.spot <= 0.0 || !std::isfinite(p.spot)) return bad("spot price must be positive");
@@ -109,15 +111,18 @@ MangoStatus validate_params(const MangoPricingParams& p, const char** why) {
     if (p.n_tenor_points == 0 && !std::isfinite(p.rate_const)) return bad("invalid rate");
     if (p.dividend_yield < 0.0 || !std::isfinite(p.dividend_yield)) return bad("invalid dividend yield");
     if (p.volatility <= 0.0 || !std::isfinite(p.volatility)) return bad("volatility must be positive");
-    if (p.n_tenor_points != 0) { *why = "yield curves are not supported by the GPU solver"; return MANGO_ERR_SOLVER; }
-    if (p.n_dividends != 0) { *why = "discrete dividends are not supported by the GPU solver"; return MANGO_ERR_SOLVER; }
+    if (events) return validate_events(p, why);
+    if (p.n_tenor_points != 0) { *why = "yield curves are not supported by this GPU entry point"; return MANGO_ERR_SOLVER; }
+    if (p.n_dividends != 0) { *why = "discrete dividends are not supported by this GPU entry point"; return MANGO_ERR_SOLVER; }
     *why = nullptr;
     return MANGO_OK;
 }
 
-// estimate_pde_grid with the step count (needs the generated grid's dx_min).
-std::pair<GridDesc, int64_t> estimate_grid_host(const MangoPricingParams& p, const Accuracy& acc) {
-    GridDesc g = estimate_spatial(p.spot, p.strike, p.maturity, p.volatility, acc);
+// estimate_pde_grid (grid_spec_types.cpp:37-118): the spatial grid (left edge
+// widened for dividends), then the step count from the generated grid's
+// dx_min and, with dividend dates, with_mandatory_points(0, T, dt_capped).
+std::pair<GridDesc, HostTimeDomain> estimate_grid_events(const MangoPricingParams& p, const Accuracy& acc) {
+    GridDesc g = estimate_spatial(p.spot, p.strike, p.maturity, p.volatility, acc, max_dividend_over_strike(p));
     std::vector<double> pts(static_cast<size_t>(g.n_points));
     generate_grid(g, pts.data());
     double dx_min = pts[1] - pts[0];
@@ -125,7 +130,15 @@ std::pair<GridDesc, int64_t> estimate_grid_host(const MangoPricingParams& p, con
     const double dt = acc.c_t * dx_min;
     int64_t nt = static_cast<int64_t>(std::ceil(p.maturity / dt));
     nt = std::min(nt, acc.max_time_steps);
-    return {g, nt};
+    const std::vector<double> mand = mandatory_taus(p);
+    if (mand.empty()) return {g, time_uniform(0.0, p.maturity, nt)};
+    const double dt_capped = std::max(dt, p.maturity / static_cast<double>(acc.max_time_steps));
+    return {g, time_with_mandatory(0.0, p.maturity, dt_capped, mand)};
+}
+
+std::pair<GridDesc, int64_t> estimate_grid_host(const MangoPricingParams& p, const Accuracy& acc) {
+    auto [g, td] = estimate_grid_events(p, acc);
+    return {g, td.n_steps};
 }
 
 std::pair<GridDesc, int64_t> estimate_batch_grid_host(const MangoPricingParams* ps, size_t n, const Accuracy& acc,
@@ -156,6 +169,18 @@ std::pair<GridDesc, int64_t> estimate_batch_grid_host(const MangoPricingParams* 
         g.n_clusters = 0;
     }
     if (max_maturity) *max_maturity = max_t;
+    // union of the dividend dates (grid_spec_types.cpp:157-175)
+    std::vector<double> mand;
+    for (size_t i = 0; i < n; ++i)
+        for (uint64_t k = 0; k < ps[i].n_dividends && ps[i].dividends; ++k) {
+            const double tau = ps[i].maturity - ps[i].dividends[k].calendar_time;
+            if (tau > 0.0 && tau < max_t) mand.push_back(tau);
+        }
+    if (!mand.empty()) {
+        const double g_dt = max_t / static_cast<double>(gnt);
+        const double dt_capped = std::max(g_dt, max_t / static_cast<double>(acc.max_time_steps));
+        gnt = time_with_mandatory(0.0, max_t, dt_capped, mand).n_steps;
+    }
     return {g, gnt};
 }
 
@@ -261,12 +286,31 @@ PdeJob* device_job_scratch(size_t n, MangoError* err) {
 
 // Run jobs on the device, one launch per kernel variant (jobs binned by grid
 // size). greeks/status are indexed by job.out_index (n_out slots).
+// Event jobs (ext != 0) binned by nodes per lane: appended to `sorted` with
+// their (nodes, start, count) ranges. False if a grid is too large.
+bool bin_event_jobs(const std::vector<PdeJob>& jobs, std::vector<PdeJob>& sorted,
+                    std::vector<std::pair<int, std::pair<size_t, size_t>>>& ranges) {
+    for (int m : {2, 4, 7}) {
+        const size_t start = sorted.size();
+        for (const auto& j : jobs) {
+            if (j.ext == 0) continue;
+            const int need = events_nodes_for(std::max<int64_t>(j.grid.n_points, 3));
+            if (need == 0) return false;
+            if (need == m) sorted.push_back(j);
+        }
+        if (sorted.size() > start) ranges.push_back({m, {start, sorted.size() - start}});
+    }
+    return true;
+}
+
 MangoStatus run_jobs(std::vector<PdeJob>& jobs, size_t n_out, Greeks* greeks, int32_t* status,
-                     HostState* state, MangoError* err) {
+                     HostState* state, MangoError* err, const std::vector<double>* pool = nullptr) {
     if (jobs.empty()) return MANGO_OK;
     if (!device_ready(err)) return MANGO_ERR_SOLVER;
     int64_t max_n = 0;
-    for (const auto& j : jobs) max_n = std::max(max_n, j.grid.n_points);
+    size_t n_plain = 0;
+    for (const auto& j : jobs)
+        if (j.ext == 0) { max_n = std::max(max_n, j.grid.n_points); ++n_plain; }
     if (max_n > max_nodes_supported()) {
         set_err(err, MANGO_ERR_SOLVER, "grid has more points than the GPU solver supports");
         return MANGO_ERR_SOLVER;
@@ -278,8 +322,9 @@ MangoStatus run_jobs(std::vector<PdeJob>& jobs, size_t n_out, Greeks* greeks, in
     {
         std::vector<SolveVariant> var(jobs.size()), all;
         for (size_t i = 0; i < jobs.size(); ++i) {
+            if (jobs[i].ext != 0) continue;
             const int64_t nsp = std::max<int64_t>(jobs[i].grid.n_points, 3);
-            var[i] = jobs.size() <= static_cast<size_t>(kLatencyJobs) ? pick_latency_variant(nsp) : pick_variant(nsp);
+            var[i] = n_plain <= static_cast<size_t>(kLatencyJobs) ? pick_latency_variant(nsp) : pick_variant(nsp);
             if (std::none_of(all.begin(), all.end(), [&](const SolveVariant& a) {
                     return a.lanes == var[i].lanes && a.nodes == var[i].nodes; }))
                 all.push_back(var[i]);
@@ -287,9 +332,20 @@ MangoStatus run_jobs(std::vector<PdeJob>& jobs, size_t n_out, Greeks* greeks, in
         for (const auto& v : all) {
             const size_t start = sorted.size();
             for (size_t i = 0; i < jobs.size(); ++i)
-                if (var[i].lanes == v.lanes && var[i].nodes == v.nodes) sorted.push_back(jobs[i]);
+                if (jobs[i].ext == 0 && var[i].lanes == v.lanes && var[i].nodes == v.nodes) sorted.push_back(jobs[i]);
             if (sorted.size() > start) ranges.push_back({v, {start, sorted.size() - start}});
         }
+    }
+    std::vector<std::pair<int, std::pair<size_t, size_t>>> ev_ranges;
+    if (!bin_event_jobs(jobs, sorted, ev_ranges)) {
+        set_err(err, MANGO_ERR_SOLVER, "grid has more points than the GPU solver supports");
+        return MANGO_ERR_SOLVER;
+    }
+    DevBuf<double> d_pool(pool ? std::max<size_t>(pool->size(), 1) : 0);
+    if (pool && !pool->empty() &&
+        (!d_pool.ok() || hipMemcpy(d_pool.p, pool->data(), pool->size() * 8, hipMemcpyHostToDevice) != hipSuccess)) {
+        set_err(err, MANGO_ERR_SOLVER, "device copy of the event records failed");
+        return MANGO_ERR_SOLVER;
     }
     const int64_t stride = state ? state->stride : 0;
     DevBuf<PdeJob> d_jobs(sorted.size());
@@ -314,6 +370,9 @@ MangoStatus run_jobs(std::vector<PdeJob>& jobs, size_t n_out, Greeks* greeks, in
     for (const auto& [v, r] : ranges)
         ok = ok && launch_american_solve(v, d_jobs.p + r.first, static_cast<int>(r.second), d_greeks.p,
                                          d_status.p, so, stream) == 0;
+    for (const auto& [m, r] : ev_ranges)
+        ok = ok && launch_american_events(m, d_jobs.p + r.first, static_cast<int>(r.second), d_pool.p, d_greeks.p,
+                                          d_status.p, so, stream) == 0;
     ok = ok && hipStreamSynchronize(stream) == hipSuccess;
     // Only the slots the jobs wrote are copied back.
     std::vector<Greeks> hg(n_out);
@@ -347,7 +406,7 @@ MangoStatus run_jobs(std::vector<PdeJob>& jobs, size_t n_out, Greeks* greeks, in
 // Normalized chain on the device: group solves with state kept in HBM, then
 // one epilogue thread per option reads its group's splines.
 MangoStatus run_chain(std::vector<PdeJob>& gjobs, const std::vector<ChainOption>& opts, size_t n_out, Greeks* greeks,
-                      int32_t* status, HostState* state, MangoError* err) {
+                      int32_t* status, HostState* state, MangoError* err, const std::vector<double>* pool = nullptr) {
     if (gjobs.empty()) return MANGO_OK;
     if (!device_ready(err)) return MANGO_ERR_SOLVER;
     int64_t stride = 0;
@@ -382,6 +441,7 @@ MangoStatus run_chain(std::vector<PdeJob>& gjobs, const std::vector<ChainOption>
         const bool few = gjobs.size() <= static_cast<size_t>(kLatencyJobs);
         auto pick = [&](int64_t nsp) { return few ? pick_latency_variant(nsp) : pick_variant(nsp); };
         for (const auto& j : gjobs) {
+            if (j.ext != 0) continue;
             const SolveVariant pv = pick(j.grid.n_points);
             if (std::none_of(all.begin(), all.end(), [&](const SolveVariant& a) {
                     return a.lanes == pv.lanes && a.nodes == pv.nodes; }))
@@ -390,16 +450,29 @@ MangoStatus run_chain(std::vector<PdeJob>& gjobs, const std::vector<ChainOption>
         for (const auto& v : all) {
             const size_t start = sorted.size();
             for (const auto& j : gjobs) {
+                if (j.ext != 0) continue;
                 const SolveVariant pv = pick(j.grid.n_points);
                 if (pv.lanes == v.lanes && pv.nodes == v.nodes) sorted.push_back(j);
             }
             if (sorted.size() > start) ranges.push_back({v, {start, sorted.size() - start}});
         }
     }
+    std::vector<std::pair<int, std::pair<size_t, size_t>>> ev_ranges;
+    if (!bin_event_jobs(gjobs, sorted, ev_ranges)) {
+        set_err(err, MANGO_ERR_SOLVER, "grid has more points than the GPU solver supports");
+        return MANGO_ERR_SOLVER;
+    }
+    DevBuf<double> d_pool(pool ? std::max<size_t>(pool->size(), 1) : 0);
+    if (pool && !pool->empty())
+        ok = ok && d_pool.ok() &&
+             hipMemcpy(d_pool.p, pool->data(), pool->size() * 8, hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemcpy(d_jobs.p, sorted.data(), ng * sizeof(PdeJob), hipMemcpyHostToDevice) == hipSuccess;
     for (const auto& [v, r] : ranges)
         ok = ok && launch_american_solve(v, d_jobs.p + r.first, static_cast<int>(r.second), nullptr, d_gstatus.p, so,
                                          stream) == 0;
+    for (const auto& [m, r] : ev_ranges)
+        ok = ok && launch_american_events(m, d_jobs.p + r.first, static_cast<int>(r.second), d_pool.p, nullptr,
+                                          d_gstatus.p, so, stream) == 0;
     ok = ok && launch_chain_epilogue(d_opts.p, static_cast<int>(opts.size()), so, d_gstatus.p, d_greeks.p,
                                      d_status.p, stream) == 0;
     ok = ok && hipStreamSynchronize(stream) == hipSuccess;
@@ -938,9 +1011,12 @@ PdeJob make_job(const MangoPricingParams& p, const GridDesc& g, int64_t n_time, 
 }
 
 // The batch planner: BatchAmericanOptionSolver::solve_batch routing.
+// single: the time domain of mango_price_american (the estimate's own,
+// AmericanOptionSolver::create(params)) instead of the batch's per-contract
+// PDEGridConfig{spec, n_steps, dividend dates}.
 MangoStatus price_batch(const MangoPricingParams* params, size_t n, bool use_shared_grid, bool use_normalized,
                         const MangoGridConfig* grid, MangoAmericanGreeks* out, int32_t* out_status,
-                        uint64_t* out_failed, MangoError* err, HostState* full_state) {
+                        uint64_t* out_failed, MangoError* err, HostState* full_state, bool single = false) {
     const double nan = std::numeric_limits<double>::quiet_NaN();
     std::vector<int32_t> status(n, MANGO_ERR_SOLVER);
     std::vector<Greeks> greeks(n, Greeks{nan, nan, nan, nan});
@@ -948,7 +1024,7 @@ MangoStatus price_batch(const MangoPricingParams* params, size_t n, bool use_sha
     std::string first_msg;
     for (size_t i = 0; i < n; ++i) {
         const char* why = nullptr;
-        status[i] = validate_params(params[i], &why);
+        status[i] = validate_params(params[i], &why, true);
         valid[i] = status[i] == MANGO_OK;
         if (!valid[i] && first_msg.empty() && why) first_msg = why;
     }
@@ -994,6 +1070,7 @@ MangoStatus price_batch(const MangoPricingParams* params, size_t n, bool use_sha
         // is_normalized_eligible
         bool elig = true;
         for (size_t i = 1; i < vp.size(); ++i) if (vp[i].option_type != vp[0].option_type) elig = false;
+        for (const auto& p : vp) if (p.n_dividends > 0) elig = false;
         if (elig) {
             const GridDesc g0 = estimate_spatial(vp[0].spot, vp[0].strike, vp[0].maturity, vp[0].volatility, acc);
             const double dx = (g0.x_max - g0.x_min) / static_cast<double>(g0.n_points - 1);
@@ -1018,7 +1095,10 @@ MangoStatus price_batch(const MangoPricingParams* params, size_t n, bool use_sha
             const auto& p = vp[i];
             bool found = false;
             for (auto& g : groups) {
-                if (std::abs(g.p.volatility - p.volatility) < TOL && std::abs(g.p.rate_const - p.rate_const) < TOL &&
+                bool rate_match = false;
+                if ((g.p.n_tenor_points > 0) == (p.n_tenor_points > 0))
+                    rate_match = p.n_tenor_points > 0 ? same_curve(g.p, p) : std::abs(g.p.rate_const - p.rate_const) < TOL;
+                if (std::abs(g.p.volatility - p.volatility) < TOL && rate_match &&
                     std::abs(g.p.dividend_yield - p.dividend_yield) < TOL &&
                     std::abs(g.p.maturity - p.maturity) < TOL && g.p.option_type == p.option_type) {
                     g.members.push_back(i);
@@ -1034,6 +1114,7 @@ MangoStatus price_batch(const MangoPricingParams* params, size_t n, bool use_sha
             }
         }
         std::vector<PdeJob> gjobs;
+        std::vector<double> pool;
         int64_t stride = 0;
         for (size_t gi = 0; gi < groups.size(); ++gi) {
             GridDesc gs;
@@ -1041,7 +1122,12 @@ MangoStatus price_batch(const MangoPricingParams* params, size_t n, bool use_sha
             if (custom) { gs = custom_spec; nt = custom_nt; }
             else { auto e = estimate_batch_grid_host(&groups[gi].p, 1, acc, nullptr); gs = e.first; nt = e.second; }
             stride = std::max(stride, gs.n_points);
-            gjobs.push_back(make_job(groups[gi].p, gs, nt, static_cast<int32_t>(gi), kJobWriteState));
+            PdeJob j = make_job(groups[gi].p, gs, nt, static_cast<int32_t>(gi), kJobWriteState);
+            if (has_events(groups[gi].p)) {
+                j.rate = 0.0;
+                j.ext = append_event_record(pool, groups[gi].p, contract_time(groups[gi].p, nt));
+            }
+            gjobs.push_back(j);
         }
         std::vector<ChainOption> copts;
         copts.reserve(vp.size());
@@ -1050,7 +1136,8 @@ MangoStatus price_batch(const MangoPricingParams* params, size_t n, bool use_sha
                 copts.push_back({vp[k].spot, vp[k].strike, static_cast<int32_t>(gi), static_cast<int32_t>(idx[k])});
         HostState st;
         st.stride = stride;
-        const MangoStatus rs = run_chain(gjobs, copts, n, greeks.data(), status.data(), full_state ? &st : nullptr, err);
+        const MangoStatus rs = run_chain(gjobs, copts, n, greeks.data(), status.data(), full_state ? &st : nullptr, err,
+                                         &pool);
         if (rs != MANGO_OK) return rs;
         if (full_state) *full_state = std::move(st);
     } else {
@@ -1066,15 +1153,24 @@ MangoStatus price_batch(const MangoPricingParams* params, size_t n, bool use_sha
             }
         }
         int64_t stride = 0;
+        std::vector<double> pool;
         for (size_t k = 0; k < vp.size(); ++k) {
             const auto& p = vp[k];
+            const bool ev = has_events(p);
             GridDesc gs;
             int64_t nt;
+            HostTimeDomain td;
             if (use_shared_grid) { gs = shared_spec; nt = shared_nt; }
             else if (custom) { gs = custom_spec; nt = custom_nt; }
             else if (mode == MANGO_GRID_MODE_ACCURACY_FIXED_TIME) {
-                gs = estimate_spatial(p.spot, p.strike, p.maturity, p.volatility, acc);
+                gs = estimate_spatial(p.spot, p.strike, p.maturity, p.volatility, acc, max_dividend_over_strike(p));
                 nt = static_cast<int64_t>(grid->n_time);
+            } else if (ev) {
+                // the estimate's step count (with the dividend dates), floored
+                auto e = estimate_grid_events(p, acc);
+                gs = e.first;
+                nt = std::max(e.second.n_steps, min_steps);
+                if (single) td = e.second;
             } else {
                 gs = estimate_spatial(p.spot, p.strike, p.maturity, p.volatility, acc);
                 nt = 0;  // derived from the generated grid on the device
@@ -1084,12 +1180,19 @@ MangoStatus price_batch(const MangoPricingParams* params, size_t n, bool use_sha
             j.c_t = acc.c_t;
             j.max_time_steps = acc.max_time_steps;
             j.min_time_steps = min_steps;
+            if (ev) {
+                if (!(single && td.n_steps > 0)) td = contract_time(p, nt);
+                j.n_time = static_cast<int32_t>(td.n_steps);
+                if (p.n_tenor_points > 0) j.rate = 0.0;
+                j.ext = append_event_record(pool, p, td);
+            }
             stride = std::max(stride, gs.n_points);
             jobs.push_back(j);
         }
         HostState st;
         st.stride = stride;
-        const MangoStatus rs = run_jobs(jobs, n, greeks.data(), status.data(), full_state ? &st : nullptr, err);
+        const MangoStatus rs = run_jobs(jobs, n, greeks.data(), status.data(), full_state ? &st : nullptr, err,
+                                        &pool);
         if (rs != MANGO_OK) return rs;
         if (full_state) *full_state = std::move(st);
     }
@@ -1152,7 +1255,7 @@ MangoStatus mango_estimate_pde_grid(const MangoPricingParams* params, const Mang
                                     MangoGridSpec* out_spec, uint64_t* out_n_time, MangoError* err) {
     if (!params || !out_spec) { set_err(err, MANGO_ERR_VALIDATION, "null argument"); return MANGO_ERR_VALIDATION; }
     const char* why = nullptr;
-    const MangoStatus s = validate_params(*params, &why);
+    const MangoStatus s = validate_params(*params, &why, true);
     if (s == MANGO_ERR_VALIDATION) { set_err(err, s, why); return s; }
     auto [g, nt] = estimate_grid_host(*params, to_accuracy(accuracy));
     from_desc(g, out_spec);
@@ -1248,13 +1351,13 @@ MangoStatus mango_price_american(const MangoPricingParams* params, MangoAmerican
     *out_result = nullptr;
     try {
         const char* why = nullptr;
-        const MangoStatus vs = validate_params(*params, &why);
+        const MangoStatus vs = validate_params(*params, &why, true);
         if (vs != MANGO_OK) { set_err(out_err, vs, why); return vs; }
         MangoAmericanGreeks g;
         int32_t st = MANGO_ERR_SOLVER;
         uint64_t failed = 0;
         HostState state;
-        const MangoStatus s = price_batch(params, 1, false, false, nullptr, &g, &st, &failed, out_err, &state);
+        const MangoStatus s = price_batch(params, 1, false, false, nullptr, &g, &st, &failed, out_err, &state, true);
         if (s != MANGO_OK) return s;
         if (st != MANGO_OK) { set_err(out_err, st, "American option solve failed"); return st; }
         auto* r = new MangoAmericanResult{};

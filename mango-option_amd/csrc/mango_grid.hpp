@@ -5,7 +5,8 @@
 //   GridSpec::multi_sinh_spaced validation + merge_nearby_clusters
 //       (src/pde/core/grid.hpp:111-165, 191-235)
 //   GridSpec::generate point formulas (src/pde/core/grid.hpp:364-498)
-//   estimate_pde_grid spatial part (src/option/grid_spec_types.cpp:37-83)
+//   estimate_pde_grid spatial part (src/option/grid_spec_types.cpp:37-83),
+//       with the dividend widening of the left edge
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -106,17 +107,22 @@ __host__ __device__ inline bool make_multi_sinh(GridDesc& g, double x_min, doubl
 }
 
 // Spatial part of estimate_pde_grid (no discrete dividends).
+// max_d_over_k: the largest D/K of the discrete dividends strictly inside
+// (0, T); the left edge moves to ln(e^x_min - D/K) so the dividend shift stays
+// on the grid, or one unit left when e^x_min <= D/K (grid_spec_types.cpp:56-69).
 __host__ __device__ inline GridDesc estimate_spatial(double spot, double strike, double maturity,
-                                                     double vol, const Accuracy& acc) {
+                                                     double vol, const Accuracy& acc, double max_d_over_k = 0.0) {
     const double sigma_sqrt_T = vol * sqrt(maturity);
     const double x0 = log(spot / strike);
-    const double x_min = x0 - acc.n_sigma * sigma_sqrt_T;
+    double x_min = x0 - acc.n_sigma * sigma_sqrt_T;
     const double x_max = x0 + acc.n_sigma * sigma_sqrt_T;
     const double dx_target = vol * sqrt(acc.tol);
     int64_t nx = static_cast<int64_t>(ceil((x_max - x_min) / dx_target));
     if (nx < acc.min_spatial_points) nx = acc.min_spatial_points;
     if (nx > acc.max_spatial_points) nx = acc.max_spatial_points;
     if (nx % 2 == 0) nx++;
+    if (max_d_over_k > 0.0 && exp(x_min) > max_d_over_k) x_min = log(exp(x_min) - max_d_over_k);
+    else if (max_d_over_k > 0.0) x_min -= 1.0;
     double c[2], a[2], w[2];
     int nc = 0;
     c[nc] = x0; a[nc] = acc.alpha; w[nc] = 0.5; ++nc;

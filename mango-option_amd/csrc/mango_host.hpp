@@ -25,9 +25,10 @@ bool device_ready(MangoError* err);
 // MangoGridAccuracy (nullable: defaults) and MangoGridSpec conversions.
 Accuracy to_accuracy(const MangoGridAccuracy* a);
 GridDesc to_desc(const MangoGridSpec& s);
-// validate_pricing_params for the device path (constant rate, no discrete
-// dividends); *why names the first violation.
-MangoStatus validate_params(const MangoPricingParams& p, const char** why);
+// validate_pricing_params; *why names the first violation. Yield curves and
+// discrete dividends are validated like the reference's with `events`, and
+// reported unsupported (MANGO_ERR_SOLVER) without.
+MangoStatus validate_params(const MangoPricingParams& p, const char** why, bool events = false);
 // estimate_batch_pde_grid (grid_spec_types.cpp:120-177): union domain,
 // largest point and step counts of the per-option estimates.
 std::pair<GridDesc, int64_t> estimate_batch_grid_host(const MangoPricingParams* ps, size_t n, const Accuracy& acc,

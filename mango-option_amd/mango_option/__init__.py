@@ -27,7 +27,7 @@ from .workloads import OptionBatch
 
 __all__ = [
     "OptionType", "GridAccuracyProfile", "GridAccuracyParams", "make_grid_accuracy", "PricingParams",
-    "Dividend", "AmericanOptionResult", "american_option_price", "BatchAmericanOptionSolver", "SolverError",
+    "Dividend", "TenorPoint", "YieldCurve", "AmericanOptionResult", "american_option_price", "BatchAmericanOptionSolver", "SolverError",
     "SolverErrorCode", "GridSpec", "grid_config", "price_batch", "estimate_pde_grid", "estimate_batch_pde_grid",
     "plan_grid_sizes",
 ]
@@ -95,12 +95,104 @@ class Dividend:
 
 
 @dataclass
+class TenorPoint:
+    """TenorPoint (src/option/yield_curve.hpp:19-23): tenor and ln D(tenor)."""
+    tenor: float = 0.0
+    log_discount: float = 0.0
+
+
+class YieldCurve:
+    """YieldCurve (src/option/yield_curve.hpp:29-180; Python names of
+    src/python/mango_bindings.cpp:344-375): log-linear interpolation of the
+    discount factor between tenors, i.e. piecewise-constant forward rates.
+    Passed as PricingParams.rate; the GPU solve reads its tenor points."""
+
+    def __init__(self, points=None):
+        self.points = [] if points is None else [(float(t), float(ld)) for t, ld in points]
+
+    @staticmethod
+    def flat(rate: float) -> "YieldCurve":
+        return YieldCurve([(0.0, 0.0), (100.0, -rate * 100.0)])
+
+    @staticmethod
+    def from_points(points) -> "YieldCurve":
+        pts = sorted(((p.tenor, p.log_discount) if isinstance(p, TenorPoint) else (float(p[0]), float(p[1]))
+                      for p in points), key=lambda q: q[0])
+        if not pts:
+            raise ValueError("Empty points vector")
+        if pts[0][0] != 0.0:
+            raise ValueError("First point must have t=0")
+        if abs(pts[0][1]) > 1e-10:
+            raise ValueError("log_discount at t=0 must be 0")
+        for i in range(1, len(pts)):
+            if pts[i][0] <= pts[i - 1][0] + 1e-10:
+                raise ValueError("Tenors must be strictly increasing")
+        return YieldCurve(pts)
+
+    @staticmethod
+    def from_discounts(tenors, discounts) -> "YieldCurve":
+        tenors, discounts = list(tenors), list(discounts)
+        if len(tenors) != len(discounts):
+            raise ValueError("Tenors and discounts must have same size")
+        if not tenors:
+            raise ValueError("Empty tenors vector")
+        if any(d <= 0.0 for d in discounts):
+            raise ValueError("Discount factors must be positive")
+        return YieldCurve.from_points([(t, math.log(d)) for t, d in zip(tenors, discounts)])
+
+    def _rate_between(self, i: int) -> float:
+        if i + 1 >= len(self.points):
+            return 0.0
+        dt = self.points[i + 1][0] - self.points[i][0]
+        return 0.0 if dt <= 0.0 else -(self.points[i + 1][1] - self.points[i][1]) / dt
+
+    def _upper(self, t: float) -> int:
+        import bisect
+        return bisect.bisect_right([q[0] for q in self.points], t)
+
+    def rate(self, t: float) -> float:
+        if len(self.points) < 2:
+            return 0.0
+        if t <= 0.0:
+            return self._rate_between(0)
+        ub = self._upper(t)
+        if ub == 0:
+            return self._rate_between(0)
+        if ub == len(self.points):
+            return self._rate_between(len(self.points) - 2)
+        return self._rate_between(ub - 1)
+
+    def log_discount(self, t: float) -> float:
+        pts = self.points
+        if len(pts) < 2 or t <= 0.0:
+            return 0.0
+        ub = self._upper(t)
+        if ub == 0:
+            return 0.0
+        if ub == len(pts):
+            r = -(pts[-1][1] - pts[-2][1]) / (pts[-1][0] - pts[-2][0])
+            return pts[-1][1] - r * (t - pts[-1][0])
+        (t0, l0), (t1, l1) = pts[ub - 1], pts[ub]
+        return l0 + (t - t0) / (t1 - t0) * (l1 - l0)
+
+    def discount(self, t: float) -> float:
+        return math.exp(self.log_discount(t))
+
+    def zero_rate(self, t: float) -> float:
+        return self.rate(0.0) if t <= 0.0 else -self.log_discount(t) / t
+
+    def __repr__(self) -> str:
+        return "<YieldCurve>"
+
+
+@dataclass
 class PricingParams:
-    """PricingParams (src/option/option_spec.hpp:262-281), constant rate."""
+    """PricingParams (src/option/option_spec.hpp:262-281): rate is a float or
+    a YieldCurve; discrete_dividends a list of Dividend."""
     spot: float = 0.0
     strike: float = 0.0
     maturity: float = 0.0
-    rate: float = 0.0
+    rate: object = 0.0
     dividend_yield: float = 0.0
     option_type: OptionType = OptionType.CALL
     volatility: float = 0.0
@@ -196,7 +288,16 @@ def _c_params(items) -> C.Array:
     for i, q in enumerate(items):
         p = arr[i]
         p.spot = q.spot; p.strike = q.strike; p.maturity = q.maturity; p.dividend_yield = q.dividend_yield
-        p.volatility = q.volatility; p.rate_const = q.rate; p.option_type = int(q.option_type)
+        p.volatility = q.volatility; p.option_type = int(q.option_type)
+        if isinstance(q.rate, YieldCurve):
+            p.rate_const = 0.0
+            pts = (_abi.MangoTenorPoint * max(1, len(q.rate.points)))(
+                *[_abi.MangoTenorPoint(t, ld) for t, ld in q.rate.points])
+            p.tenor_points = C.cast(pts, C.POINTER(_abi.MangoTenorPoint))
+            p.n_tenor_points = len(q.rate.points)
+            setattr(arr, f"_curve{i}", pts)
+        else:
+            p.rate_const = q.rate
         if q.discrete_dividends:
             divs = (_abi.MangoDividend * len(q.discrete_dividends))(
                 *[_abi.MangoDividend(d.calendar_time, d.amount) for d in q.discrete_dividends])
@@ -350,7 +451,7 @@ def _validation_message(p: PricingParams) -> str | None:
         return f"Invalid pricing parameters: maturity must be positive (value={p.maturity})"
     if not (p.volatility > 0 and math.isfinite(p.volatility)):
         return f"Invalid pricing parameters: volatility must be positive (value={p.volatility})"
-    if not math.isfinite(p.rate):
+    if not isinstance(p.rate, YieldCurve) and not math.isfinite(p.rate):
         return f"Invalid pricing parameters: invalid rate (value={p.rate})"
     if not (p.dividend_yield >= 0 and math.isfinite(p.dividend_yield)):
         return f"Invalid pricing parameters: invalid dividend yield (value={p.dividend_yield})"

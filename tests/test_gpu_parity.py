@@ -160,11 +160,12 @@ def test_invalid_and_unsupported_slots():
     good = [0, 2, 4, 5]
     gr, sr, _, _ = O.price_batch(opts.take(good), False, c1_grid(), use_normalized=False)
     _check(g[good], s[good], gr, sr, "valid-subset")
-    # discrete dividends are rejected loudly (no CPU fallback)
+    # a discrete dividend after maturity is a validation failure of its slot only
     p = M.PricingParams(spot=100, strike=100, maturity=1, rate=0.05, option_type=M.OptionType.PUT, volatility=0.2,
-                        discrete_dividends=[M.Dividend(0.5, 1.0)])
-    g, s, failed = M.price_batch([p], False, None)
-    assert failed == 1 and s[0] == _abi.MANGO_ERR_SOLVER
+                        discrete_dividends=[M.Dividend(1.5, 1.0)])
+    q = M.PricingParams(spot=100, strike=100, maturity=1, rate=0.05, option_type=M.OptionType.PUT, volatility=0.2)
+    g, s, failed = M.price_batch([p, q], False, None)
+    assert failed == 1 and s[0] == _abi.MANGO_ERR_VALIDATION and s[1] == _abi.MANGO_OK
 
 
 def test_large_grids_401():
