@@ -22,6 +22,8 @@ Also measured in the same run:
 - price table: the 50 x 30 x 20 x 10 EEP B-spline table built on the GPU
   from 200 snapshot PDE solves, and 1,048,576 interpolated-IV queries on it
   (mango_price_table_build, mango_interp_iv_solve_device);
+- events: 16,384 puts with a yield curve and two cash dividends each through
+  the host-buffer batch entry (mango_price_american_batch_ex, event kernel);
 - cpu_baseline: the oracle (CPU restatement of the reference's OpenMP batch
   path) on the host cores, which also checks a sample of the GPU results.
 """
@@ -299,7 +301,60 @@ def run_table(g: Gpu, args):
             "_rows": rec[:k], "_iv": res[:k], "_coeffs": coeffs}
 
 
-def cpu_baseline(chains, batched, seconds: float, adjoint=None, table=None):
+EVENT_CURVE = [(0.0, 0.0), (0.25, -0.0075), (0.75, -0.0275), (2.0, -0.09)]  # 3%, 4%, 5% forwards
+
+
+def event_params(g_abi, batch):
+    """MangoPricingParams[n] of `batch` with one yield curve (EVENT_CURVE) and
+    two cash dividends per option (1% of spot at 0.3 T and 0.7 T); returns
+    the array and the arrays it points into."""
+    n = len(batch)
+    arr = (g_abi.MangoPricingParams * n)()
+    curve = (g_abi.MangoTenorPoint * len(EVENT_CURVE))(*[g_abi.MangoTenorPoint(a, b) for a, b in EVENT_CURVE])
+    divs = (g_abi.MangoDividend * (2 * n))()
+    for i in range(n):
+        p = arr[i]
+        p.spot = float(batch.spot[i]); p.strike = float(batch.strike[i]); p.maturity = float(batch.maturity[i])
+        p.dividend_yield = float(batch.dividend_yield[i]); p.volatility = float(batch.volatility[i])
+        p.option_type = int(batch.option_type[i]); p.rate_const = 0.0
+        p.tenor_points = C.cast(curve, C.POINTER(g_abi.MangoTenorPoint)); p.n_tenor_points = len(EVENT_CURVE)
+        divs[2 * i] = g_abi.MangoDividend(0.3 * p.maturity, 0.01 * p.spot)
+        divs[2 * i + 1] = g_abi.MangoDividend(0.7 * p.maturity, 0.01 * p.spot)
+        p.dividends = C.cast(C.byref(divs, 2 * i * C.sizeof(g_abi.MangoDividend)), C.POINTER(g_abi.MangoDividend))
+        p.n_dividends = 2
+    return arr, (curve, divs)
+
+
+def run_events(g: Gpu, args):
+    """Yield curve + two discrete dividends per option through the host-buffer
+    batch entry (event kernel); wall time includes host planning and PCIe."""
+    base = g.W.c1_puts(min(args.event_options, 4096), seed=0)
+    batch = g.W.replicate(base, args.event_options)
+    n = len(batch)
+    arr, keep = event_params(g.abi, batch)
+    cfg = chain_grid_config(g.M)
+    out = (g.abi.MangoAmericanGreeks * n)()
+    st = (C.c_int32 * n)()
+    failed = C.c_uint64()
+
+    def step():
+        g.check(g.lib.mango_price_american_batch_ex(arr, n, 0, 0, C.byref(cfg), out, st, C.byref(failed),
+                                                    C.byref(g.err)))
+
+    el, kms, launches = g.timed(step, args.event_steps, 1)
+    if failed.value:
+        raise RuntimeError(f"{failed.value} event options failed")
+    greeks = np.frombuffer(out, dtype=np.float64).reshape(n, 4)
+    ach = n * float(flops_per_option(N_SPACE, N_TIME)) / (kms * 1e-3) / 1e12
+    return {"workload": f"{n} puts of the plain configuration distribution with a 4-tenor yield curve and two "
+                        f"cash dividends each, 101-point grids x 498 steps (+ the dividend dates), host buffers",
+            "options": n, "options_per_s": n * args.event_steps / el, "ms_per_step": 1e3 * el / args.event_steps,
+            "kernel_ms": kms, "kernel_options_per_s": n / (kms * 1e-3),
+            "roofline": {"achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS},
+            "_arr": arr, "_keep": keep, "_cfg": cfg, "_greeks": greeks[:args.event_check].copy()}
+
+
+def cpu_baseline(chains, batched, seconds: float, adjoint=None, table=None, events=None):
     """The oracle (CPU restatement of the reference) pricing whole chains the
     reference's way: one solve_batch(group, use_shared_grid=true, grid) per
     (chain, type) with the chain's union grid; OpenMP across the regular
@@ -361,6 +416,15 @@ def cpu_baseline(chains, batched, seconds: float, adjoint=None, table=None):
                         "sample": f"one full table build and {len(rows)} of the IV queries on the oracle's table",
                         "parity_iv_status_mismatches": int(np.sum(gst != rst)),
                         "parity_max_abs_iv_err": float(np.max(np.abs(giv[ok] - riv[ok]), initial=0.0))}
+    if events is not None:
+        k = len(events["_greeks"])
+        t0 = time.perf_counter()
+        gr, _, _, _ = O.price_batch(None, False, events["_cfg"], use_normalized=False, n_threads=threads,
+                                    params_array=(events["_arr"], k))
+        ev_s = time.perf_counter() - t0
+        out["events"] = {"value": k / ev_s, "unit": "options/s", "cores": threads,
+                         "sample": f"the first {k} options of the event workload, oracle OpenMP batch",
+                         "parity_max_abs_price_err": float(np.max(np.abs(gr[:, 0] - events["_greeks"][:, 0])))}
     if adjoint is not None:
         ref, _ = O.adjoint_batch(adjoint["_batch"], adjoint["_cfg"], n_threads=threads)
         sens = adjoint["_out"][:, 4:8]
@@ -387,6 +451,9 @@ def main():
     ap.add_argument("--table-iv-steps", type=int, default=5)
     ap.add_argument("--table-check", type=int, default=65536, help="IV queries checked against the oracle")
     ap.add_argument("--no-table", action="store_true")
+    ap.add_argument("--event-options", type=int, default=16384)
+    ap.add_argument("--event-steps", type=int, default=3)
+    ap.add_argument("--event-check", type=int, default=512, help="event options checked against the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", default="", help="force lanes,nodes_per_lane of the solve kernel at 101 points")
@@ -402,6 +469,7 @@ def main():
     batched = run_batched(g, args) if args.batched_options > 0 else None
     adjoint = run_adjoint(g, args) if args.adjoint_options > 0 else None
     table = None if args.no_table else run_table(g, args)
+    events = run_events(g, args) if args.event_options > 0 else None
 
     workload = "chains_4096_101x498"
     flops_launch = chains["pde_solves"] * float(flops_per_option(N_SPACE, N_TIME))
@@ -436,7 +504,10 @@ def main():
         line["adjoint"] = {k: v for k, v in adjoint.items() if not k.startswith("_")}
     if table is not None:
         line["table"] = {k: v for k, v in table.items() if not k.startswith("_")}
-    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(chains, batched, args.cpu_seconds, adjoint, table)
+    if events is not None:
+        line["events"] = {k: v for k, v in events.items() if not k.startswith("_")}
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(chains, batched, args.cpu_seconds, adjoint, table,
+                                                                 events)
     print(json.dumps(line), flush=True)
 
 

@@ -13,8 +13,8 @@
 //   dividend shift u(x) <- s_u(ln(e^x - D/K))          src/option/american_option.cpp:79-144
 //   curve forward rate / log-linear discount           src/option/yield_curve.hpp:104-178
 //
-// One job per wavefront (64 lanes x M nodes, the solve kernel's layout). The
-// stage factorizations are cached per stage kind while the weight and the
+// One job per group of K lanes x M nodes (the solve kernel's layout; 64 / K
+// groups per wavefront). The stage factorizations are cached per stage kind while the weight and the
 // rate are unchanged (piecewise-constant forward rates change them only at
 // tenors, dt only at mandatory points). A dividend writes the state to LDS,
 // lane 0 fits the natural spline, and every lane evaluates its nodes' shifted
@@ -24,8 +24,6 @@
 
 namespace mango_amd {
 namespace {
-
-constexpr int kEventLanes = 64;
 
 // One job's record in the pool (layout documented in mango_events.hpp).
 struct EventRecord {
@@ -93,12 +91,52 @@ struct EventRecord {
     __device__ double discount(double t) const { return exp(log_discount(t)); }
 };
 
-template <int M>
+// The curve around the last calendar time asked for: the forward rate and
+// the log-discount bracket are constant / linear on [s_lo, s_hi), so most
+// stages reuse them without searching the tenors (same values as
+// EventRecord::curve_rate / log_discount at every s).
+struct CurveCursor {
+    double s_lo = 1.0, s_hi = 0.0;  // empty: the first query locates
+    double rate = 0.0;
+    int ub = 0;  // first tenor > s (std::upper_bound), 0 for a curve of < 2 points
+    double t0 = 0.0, t1 = 0.0, l0 = 0.0, l1 = 0.0, r_last = 0.0;
+
+    __device__ void locate(const EventRecord& rec, double s) {
+        const int n = rec.n_curve;
+        if (n < 2) {
+            ub = 0; rate = 0.0; s_lo = -INFINITY; s_hi = INFINITY;
+            return;
+        }
+        ub = rec.upper(s);
+        rate = rec.curve_rate(s);
+        s_lo = (ub <= 1) ? -INFINITY : rec.tenor[ub - 1];
+        s_hi = (ub >= n) ? INFINITY : rec.tenor[ub];
+        if (ub >= n) {
+            t1 = rec.tenor[n - 1];
+            l1 = rec.logd[n - 1];
+            r_last = -(rec.logd[n - 1] - rec.logd[n - 2]) / (rec.tenor[n - 1] - rec.tenor[n - 2]);
+        } else if (ub >= 1) {
+            t0 = rec.tenor[ub - 1]; t1 = rec.tenor[ub];
+            l0 = rec.logd[ub - 1]; l1 = rec.logd[ub];
+        }
+    }
+    __device__ __forceinline__ void at(const EventRecord& rec, double s) {
+        if (!(s >= s_lo && s < s_hi)) locate(rec, s);
+    }
+    // YieldCurve::log_discount(s) for the located bracket
+    __device__ __forceinline__ double log_discount(int n, double s) const {
+        if (n < 2 || s <= 0.0 || ub == 0) return 0.0;
+        if (ub >= n) return l1 - r_last * (s - t1);
+        const double a = (s - t0) / (t1 - t0);
+        return l0 + a * (l1 - l0);
+    }
+};
+
+template <int K, int M>
 __device__ void solve_event_job(const PdeJob& job, const double* __restrict__ pool, const int lane,
                                 double* __restrict__ xs, double* __restrict__ ys, double* __restrict__ cps,
                                 double* __restrict__ cs, Greeks* __restrict__ greeks_out,
                                 int32_t* __restrict__ status_out, const StateOut& state) {
-    constexpr int K = kEventLanes;
     OptionPde<K, M> P;
     P.init(job, lane, xs);
     EventRecord rec;
@@ -108,15 +146,27 @@ __device__ void solve_event_job(const PdeJob& job, const double* __restrict__ po
     const int n = P.n;
     const int base = P.base;
 
-    auto rate_at = [&](double tau) { return rec.n_curve ? rec.curve_rate(smax(0.0, T - tau)) : job.rate; };
+    CurveCursor cur;
+    const double d_T = rec.n_curve ? rec.discount(T) : 1.0;
+    auto rate_at = [&](double tau) {
+        if (!rec.n_curve) return job.rate;
+        cur.at(rec, smax(0.0, T - tau));
+        return cur.rate;
+    };
     auto g_right = [&](double tau) -> double {
         if (put) return 0.0;
         if (!rec.n_curve) return P.ex_last - exp(-job.rate * tau);
         const double s = smax(0.0, T - tau);
-        return P.ex_last - rec.discount(T) / rec.discount(s);
+        cur.at(rec, s);
+        return P.ex_last - d_T / exp(cur.log_discount(rec.n_curve, s));
     };
 
-    // factorization caches: valid while (weight, band version) are unchanged
+    // Factorization caches, valid while the band version and the weight are
+    // unchanged. Inside a segment of a mandatory-point time domain the step
+    // sizes t_{i+1} - t_i differ only by the rounding of seg_start + j sub_dt,
+    // so a weight within 1e-13 (relative) of the cached one reuses the cached
+    // c' and 1/denom; the right-hand side and the lower band still use the
+    // step's own weight.
     StageCache<M> c0, c1;
     c0.invalidate();
     c1.invalidate();
@@ -126,7 +176,7 @@ __device__ void solve_event_job(const PdeJob& job, const double* __restrict__ po
         if (r != P.r) { P.set_rate(r, xs); ++ver; }
     };
     auto use_cache = [&](StageCache<M>& c, double& cw, int& cv, double w) {
-        if (w != cw || cv != ver) { c.invalidate(); cw = w; cv = ver; }
+        if (!(fabs(w - cw) <= 1e-13 * fabs(cw)) || cv != ver) { c.invalidate(); cw = w; cv = ver; }
     };
 
     // discrete dividend at time-to-expiry te, then boundary values and obstacle
@@ -168,24 +218,31 @@ __device__ void solve_event_job(const PdeJob& job, const double* __restrict__ po
         }
     };
     int ev = 0;
+    double next_tau = rec.n_ev > 0 ? rec.ev_tau[0] : INFINITY;  // time of event `ev`
     auto process_events = [&](double t_old, double t_new, double (&u)[M]) {
-        while (ev < rec.n_ev) {
-            const double te = rec.ev_tau[ev];
-            if (te <= t_old) { ++ev; continue; }
-            if (te > t_new) break;
-            apply_dividend(rec.ev_d[ev], te, u);
+        while (ev < rec.n_ev && next_tau <= t_new) {
+            if (next_tau > t_old) apply_dividend(rec.ev_d[ev], next_tau, u);
             ++ev;
+            next_tau = ev < rec.n_ev ? rec.ev_tau[ev] : INFINITY;
         }
     };
 
     // time points: uniform t_i = i dt, or seg_start + j sub_dt per segment
+    // (the current segment's constants held in registers)
     int seg = 0;
     int64_t sub = 0;
+    double seg0 = 0.0, seg_dt = 0.0;
+    int64_t seg_n = 0;
+    if (rec.n_seg > 0) { seg0 = rec.seg_start[0]; seg_dt = rec.sub_dt[0]; seg_n = static_cast<int64_t>(rec.n_sub[0]); }
     auto next_point = [&]() -> double {
         ++sub;
         if (rec.n_seg == 0) return static_cast<double>(sub) * rec.dt_avg;
-        if (sub > static_cast<int64_t>(rec.n_sub[seg])) { ++seg; sub = 1; }
-        return rec.seg_start[seg] + static_cast<double>(sub) * rec.sub_dt[seg];
+        if (sub > seg_n) {
+            ++seg;
+            sub = 1;
+            seg0 = rec.seg_start[seg]; seg_dt = rec.sub_dt[seg]; seg_n = static_cast<int64_t>(rec.n_sub[seg]);
+        }
+        return seg0 + static_cast<double>(sub) * seg_dt;
     };
 
     double u[M], up[M];
@@ -305,50 +362,58 @@ __device__ void solve_event_job(const PdeJob& job, const double* __restrict__ po
     group_sync();
 }
 
-template <int M>
-__global__ __launch_bounds__(kEventLanes) void american_events_kernel(const PdeJob* __restrict__ jobs, int n_jobs,
+template <int K, int M>
+__global__ __launch_bounds__(kSolveBlock) void american_events_kernel(const PdeJob* __restrict__ jobs, int n_jobs,
                                                                       const double* __restrict__ pool,
                                                                       Greeks* __restrict__ greeks_out,
                                                                       int32_t* __restrict__ status_out,
                                                                       StateOut state) {
-    constexpr int NMAX = kEventLanes * M;
-    __shared__ double s_x[NMAX];
-    __shared__ double s_y[NMAX];
-    __shared__ double s_cp[NMAX];
-    __shared__ double s_c[NMAX];
-    const int lane = threadIdx.x;
-    // one job per block; the loop condition is uniform over the wavefront
-    for (int j = blockIdx.x; j < n_jobs; j += gridDim.x)
-        solve_event_job<M>(jobs[j], pool, lane, s_x, s_y, s_cp, s_c, greeks_out, status_out, state);
+    constexpr int NMAX = K * M;
+    constexpr int GPB = kSolveBlock / K;
+    __shared__ double s_x[GPB][NMAX];
+    __shared__ double s_y[GPB][NMAX];
+    __shared__ double s_cp[GPB][NMAX];
+    __shared__ double s_c[GPB][NMAX];
+    const int slot = threadIdx.x / K;
+    const int lane = threadIdx.x % K;
+    // grid-stride over the jobs; the loop condition is uniform within a group
+    for (int j = blockIdx.x * GPB + slot; j < n_jobs; j += gridDim.x * GPB)
+        solve_event_job<K, M>(jobs[j], pool, lane, s_x[slot], s_y[slot], s_cp[slot], s_c[slot], greeks_out,
+                              status_out, state);
 }
 
 }  // namespace
 
-int events_nodes_for(int64_t n_points) {
-    if (n_points <= kEventLanes * 2) return 2;
-    if (n_points <= kEventLanes * 4) return 4;
-    if (n_points <= kEventLanes * 7) return 7;
-    return 0;
+// (lanes, nodes per lane) of an event launch: several jobs per wavefront for
+// launches that fill the chip, one job per wavefront for small ones.
+SolveVariant events_variant_for(int64_t n_points, int64_t n_jobs) {
+    if (n_jobs > kLatencyJobs) {
+        if (n_points <= 16 * 7) return {16, 7};
+        if (n_points <= 32 * 4) return {32, 4};
+    }
+    if (n_points <= 64 * 2) return {64, 2};
+    if (n_points <= 64 * 4) return {64, 4};
+    if (n_points <= 64 * 7) return {64, 7};
+    return {0, 0};
 }
 
-int launch_american_events(int nodes_per_lane, const PdeJob* d_jobs, int n_jobs, const double* d_pool,
+#define MANGO_EVENT_VARIANTS(X) X(16, 7) X(32, 4) X(64, 2) X(64, 4) X(64, 7)
+
+int launch_american_events(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, const double* d_pool,
                            Greeks* d_greeks, int32_t* d_status, const StateOut& state, hipStream_t stream) {
     if (n_jobs <= 0) return 0;
-    const int blocks = n_jobs < 65535 ? n_jobs : 65535;
-    switch (nodes_per_lane) {
-        case 2:
-            american_events_kernel<2><<<blocks, kEventLanes, 0, stream>>>(d_jobs, n_jobs, d_pool, d_greeks, d_status, state);
-            break;
-        case 4:
-            american_events_kernel<4><<<blocks, kEventLanes, 0, stream>>>(d_jobs, n_jobs, d_pool, d_greeks, d_status, state);
-            break;
-        case 7:
-            american_events_kernel<7><<<blocks, kEventLanes, 0, stream>>>(d_jobs, n_jobs, d_pool, d_greeks, d_status, state);
-            break;
-        default:
-            return 1;
+    const int gpb = kSolveBlock / v.lanes;
+    int blocks = (n_jobs + gpb - 1) / gpb;
+    if (blocks > 65535) blocks = 65535;
+#define X(KK, MM)                                                                                                \
+    if (v.lanes == KK && v.nodes == MM) {                                                                        \
+        american_events_kernel<KK, MM><<<blocks, kSolveBlock, 0, stream>>>(d_jobs, n_jobs, d_pool, d_greeks,    \
+                                                                            d_status, state);                    \
+        return hipGetLastError() == hipSuccess ? 0 : 1;                                                          \
     }
-    return hipGetLastError() == hipSuccess ? 0 : 1;
+    MANGO_EVENT_VARIANTS(X)
+#undef X
+    return 1;
 }
 
 }  // namespace mango_amd

@@ -286,19 +286,29 @@ PdeJob* device_job_scratch(size_t n, MangoError* err) {
 
 // Run jobs on the device, one launch per kernel variant (jobs binned by grid
 // size). greeks/status are indexed by job.out_index (n_out slots).
-// Event jobs (ext != 0) binned by nodes per lane: appended to `sorted` with
-// their (nodes, start, count) ranges. False if a grid is too large.
+int timed_events(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, const double* d_pool, Greeks* d_greeks,
+                 int32_t* d_status, const StateOut& state, hipStream_t stream);
+
+// Event jobs (ext != 0) binned by variant: appended to `sorted` with their
+// (variant, start, count) ranges. False if a grid is too large.
 bool bin_event_jobs(const std::vector<PdeJob>& jobs, std::vector<PdeJob>& sorted,
-                    std::vector<std::pair<int, std::pair<size_t, size_t>>>& ranges) {
-    for (int m : {2, 4, 7}) {
+                    std::vector<std::pair<SolveVariant, std::pair<size_t, size_t>>>& ranges) {
+    int64_t n_ev = 0;
+    for (const auto& j : jobs) n_ev += j.ext != 0;
+    std::vector<SolveVariant> var(jobs.size()), all;
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        if (jobs[i].ext == 0) continue;
+        var[i] = events_variant_for(std::max<int64_t>(jobs[i].grid.n_points, 3), n_ev);
+        if (var[i].lanes == 0) return false;
+        if (std::none_of(all.begin(), all.end(), [&](const SolveVariant& a) {
+                return a.lanes == var[i].lanes && a.nodes == var[i].nodes; }))
+            all.push_back(var[i]);
+    }
+    for (const auto& v : all) {
         const size_t start = sorted.size();
-        for (const auto& j : jobs) {
-            if (j.ext == 0) continue;
-            const int need = events_nodes_for(std::max<int64_t>(j.grid.n_points, 3));
-            if (need == 0) return false;
-            if (need == m) sorted.push_back(j);
-        }
-        if (sorted.size() > start) ranges.push_back({m, {start, sorted.size() - start}});
+        for (size_t i = 0; i < jobs.size(); ++i)
+            if (jobs[i].ext != 0 && var[i].lanes == v.lanes && var[i].nodes == v.nodes) sorted.push_back(jobs[i]);
+        ranges.push_back({v, {start, sorted.size() - start}});
     }
     return true;
 }
@@ -336,7 +346,7 @@ MangoStatus run_jobs(std::vector<PdeJob>& jobs, size_t n_out, Greeks* greeks, in
             if (sorted.size() > start) ranges.push_back({v, {start, sorted.size() - start}});
         }
     }
-    std::vector<std::pair<int, std::pair<size_t, size_t>>> ev_ranges;
+    std::vector<std::pair<SolveVariant, std::pair<size_t, size_t>>> ev_ranges;
     if (!bin_event_jobs(jobs, sorted, ev_ranges)) {
         set_err(err, MANGO_ERR_SOLVER, "grid has more points than the GPU solver supports");
         return MANGO_ERR_SOLVER;
@@ -371,8 +381,8 @@ MangoStatus run_jobs(std::vector<PdeJob>& jobs, size_t n_out, Greeks* greeks, in
         ok = ok && launch_american_solve(v, d_jobs.p + r.first, static_cast<int>(r.second), d_greeks.p,
                                          d_status.p, so, stream) == 0;
     for (const auto& [m, r] : ev_ranges)
-        ok = ok && launch_american_events(m, d_jobs.p + r.first, static_cast<int>(r.second), d_pool.p, d_greeks.p,
-                                          d_status.p, so, stream) == 0;
+        ok = ok && timed_events(m, d_jobs.p + r.first, static_cast<int>(r.second), d_pool.p, d_greeks.p,
+                                d_status.p, so, stream) == 0;
     ok = ok && hipStreamSynchronize(stream) == hipSuccess;
     // Only the slots the jobs wrote are copied back.
     std::vector<Greeks> hg(n_out);
@@ -457,7 +467,7 @@ MangoStatus run_chain(std::vector<PdeJob>& gjobs, const std::vector<ChainOption>
             if (sorted.size() > start) ranges.push_back({v, {start, sorted.size() - start}});
         }
     }
-    std::vector<std::pair<int, std::pair<size_t, size_t>>> ev_ranges;
+    std::vector<std::pair<SolveVariant, std::pair<size_t, size_t>>> ev_ranges;
     if (!bin_event_jobs(gjobs, sorted, ev_ranges)) {
         set_err(err, MANGO_ERR_SOLVER, "grid has more points than the GPU solver supports");
         return MANGO_ERR_SOLVER;
@@ -471,8 +481,8 @@ MangoStatus run_chain(std::vector<PdeJob>& gjobs, const std::vector<ChainOption>
         ok = ok && launch_american_solve(v, d_jobs.p + r.first, static_cast<int>(r.second), nullptr, d_gstatus.p, so,
                                          stream) == 0;
     for (const auto& [m, r] : ev_ranges)
-        ok = ok && launch_american_events(m, d_jobs.p + r.first, static_cast<int>(r.second), d_pool.p, nullptr,
-                                          d_gstatus.p, so, stream) == 0;
+        ok = ok && timed_events(m, d_jobs.p + r.first, static_cast<int>(r.second), d_pool.p, nullptr,
+                                d_gstatus.p, so, stream) == 0;
     ok = ok && launch_chain_epilogue(d_opts.p, static_cast<int>(opts.size()), so, d_gstatus.p, d_greeks.p,
                                      d_status.p, stream) == 0;
     ok = ok && hipStreamSynchronize(stream) == hipSuccess;
@@ -596,6 +606,22 @@ int timed_solve_ex(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, Gree
 int timed_solve(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, Greeks* d_greeks, int32_t* d_status,
                 const StateOut& state, hipStream_t stream, const int32_t* d_n_jobs) {
     return timed_solve_ex(v, d_jobs, n_jobs, d_greeks, d_status, state, stream, d_n_jobs, nullptr, nullptr, nullptr);
+}
+
+int timed_events(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, const double* d_pool, Greeks* d_greeks,
+                 int32_t* d_status, const StateOut& state, hipStream_t stream) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (g_timing) {
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        (void)hipEventRecord(a, stream);
+    }
+    const int rc = launch_american_events(v, d_jobs, n_jobs, d_pool, d_greeks, d_status, state, stream);
+    if (g_timing) {
+        (void)hipEventRecord(b, stream);
+        g_events.push_back({a, b});
+    }
+    return rc;
 }
 
 // Grow-only device scratch for the device-resident chain path.

@@ -59,11 +59,11 @@ HostTimeDomain contract_time(const MangoPricingParams& p, int64_t n_time);
 // Returns 1 + o (the PdeJob::ext value).
 int64_t append_event_record(std::vector<double>& pool, const MangoPricingParams& p, const HostTimeDomain& td);
 
-// Event kernel: one job per wavefront of 64 lanes holding M nodes each;
-// jobs must have ext != 0 and n_points <= 64 * M for the chosen M.
-int launch_american_events(int nodes_per_lane, const PdeJob* d_jobs, int n_jobs, const double* d_pool,
+// Event kernel: one job per group of v.lanes lanes holding v.nodes nodes
+// each; jobs must have ext != 0 and n_points <= lanes * nodes.
+int launch_american_events(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, const double* d_pool,
                            Greeks* d_greeks, int32_t* d_status, const StateOut& state, hipStream_t stream);
-// Smallest supported nodes-per-lane for n points (0 if none).
-int events_nodes_for(int64_t n_points);
+// Variant for n points in a launch of n_jobs jobs ({0, 0} if n is too large).
+SolveVariant events_variant_for(int64_t n_points, int64_t n_jobs);
 
 }  // namespace mango_amd

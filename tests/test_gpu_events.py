@@ -165,3 +165,26 @@ def test_invalid_curves_and_dividends_fail_per_slot():
     assert np.isnan(g[1:]).all()
     with pytest.raises(ValueError):
         M.american_option_price(bad_div)
+
+
+def test_large_event_batch_uses_packed_groups():
+    # more than 1024 event jobs take the several-jobs-per-wavefront variants
+    _say("test_large_event_batch_uses_packed_groups: 1100 puts/calls with a curve and a dividend, 101x300")
+    rng = np.random.default_rng(21)
+    cases = []
+    for i in range(1100):
+        T = float(rng.uniform(0.2, 1.5))
+        cases.append((float(rng.uniform(80, 120)), 100.0, T, float(rng.uniform(0.15, 0.45)), 0.0,
+                      float(rng.uniform(0.0, 0.03)), bool(i % 2), CURVE_STEP, [(float(rng.uniform(0.05, 0.95)) * T, 1.5)]))
+    mp = [_mp(*c) for c in cases]
+    op = [_op(*c) for c in cases]
+    acc = M.GridAccuracyParams(min_spatial_points=101, max_spatial_points=101)
+    for n_pts in (101, 121):
+        acc = M.GridAccuracyParams(min_spatial_points=n_pts, max_spatial_points=n_pts)
+        cfg = M.grid_config(_abi.MANGO_GRID_MODE_ACCURACY_FIXED_TIME, acc, n_time=300)
+        g, s, failed = M.price_batch(mp, False, cfg, use_normalized=False)
+        gr, sr, fr, _ = O.price_batch(None, False, cfg, use_normalized=False, n_threads=8,
+                                      params_array=O.params_array(op))
+        assert failed == fr == 0
+        err = _check(g, s, gr, sr, f"{n_pts} points")
+        _say(f"  {n_pts} points: max |dprice| = {err:.3e}")
