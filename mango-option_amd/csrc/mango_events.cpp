@@ -69,16 +69,46 @@ std::vector<MangoTenorPoint> sorted_curve(const MangoPricingParams& p) {
 }
 }  // namespace
 
+bool validate_curve(const MangoTenorPoint* pts, uint64_t n, const char** why) {
+    if (n == 0) return true;
+    if (!pts) { *why = "tenor_points is null but n_tenor_points > 0"; return false; }
+    std::vector<MangoTenorPoint> v(pts, pts + n);
+    std::sort(v.begin(), v.end(), [](const MangoTenorPoint& a, const MangoTenorPoint& b) { return a.tenor < b.tenor; });
+    if (v[0].tenor != 0.0) { *why = "invalid yield curve: First point must have t=0"; return false; }
+    if (std::abs(v[0].log_discount) > 1e-10) { *why = "invalid yield curve: log_discount at t=0 must be 0"; return false; }
+    for (size_t i = 1; i < v.size(); ++i)
+        if (v[i].tenor <= v[i - 1].tenor + 1e-10) { *why = "invalid yield curve: Tenors must be strictly increasing"; return false; }
+    return true;
+}
+
+double curve_zero_rate(const MangoTenorPoint* pts, uint64_t n, double T) {
+    std::vector<MangoTenorPoint> v(pts, pts + n);
+    std::sort(v.begin(), v.end(), [](const MangoTenorPoint& a, const MangoTenorPoint& b) { return a.tenor < b.tenor; });
+    const size_t m = v.size();
+    auto rate_between = [&](size_t i) {
+        if (i + 1 >= m) return 0.0;
+        const double dt = v[i + 1].tenor - v[i].tenor;
+        return dt <= 0.0 ? 0.0 : -(v[i + 1].log_discount - v[i].log_discount) / dt;
+    };
+    if (T <= 0.0) return m < 2 ? 0.0 : rate_between(0);
+    if (m < 2) return -0.0 / T;
+    const size_t ub = static_cast<size_t>(std::upper_bound(v.begin(), v.end(), T, [](double t, const MangoTenorPoint& q) {
+                                              return t < q.tenor; }) - v.begin());
+    double ld;
+    if (ub == 0) ld = 0.0;
+    else if (ub == m) {
+        const double r = -(v[m - 1].log_discount - v[m - 2].log_discount) / (v[m - 1].tenor - v[m - 2].tenor);
+        ld = v[m - 1].log_discount - r * (T - v[m - 1].tenor);
+    } else {
+        const double a = (T - v[ub - 1].tenor) / (v[ub].tenor - v[ub - 1].tenor);
+        ld = v[ub - 1].log_discount + a * (v[ub].log_discount - v[ub - 1].log_discount);
+    }
+    return -ld / T;
+}
+
 MangoStatus validate_events(const MangoPricingParams& p, const char** why) {
     auto bad = [&](const char* m) { *why = m; return MANGO_ERR_VALIDATION; };
-    if (p.n_tenor_points > 0) {
-        if (!p.tenor_points) return bad("tenor_points is null but n_tenor_points > 0");
-        const auto v = sorted_curve(p);
-        if (v[0].tenor != 0.0) return bad("invalid yield curve: First point must have t=0");
-        if (std::abs(v[0].log_discount) > 1e-10) return bad("invalid yield curve: log_discount at t=0 must be 0");
-        for (size_t i = 1; i < v.size(); ++i)
-            if (v[i].tenor <= v[i - 1].tenor + 1e-10) return bad("invalid yield curve: Tenors must be strictly increasing");
-    }
+    if (p.n_tenor_points > 0 && !validate_curve(p.tenor_points, p.n_tenor_points, why)) return MANGO_ERR_VALIDATION;
     if (p.n_dividends > 0 && !p.dividends) return bad("dividends is null but n_dividends > 0");
     for (uint64_t i = 0; i < p.n_dividends; ++i) {
         const MangoDividend& d = p.dividends[i];

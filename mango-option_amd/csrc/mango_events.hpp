@@ -44,6 +44,11 @@ double max_dividend_over_strike(const MangoPricingParams& p);
 // The curve and dividend checks of mango_c_api.cpp:42-78 and
 // validate_pricing_params (option_spec.cpp:131-152); *why is static text.
 MangoStatus validate_events(const MangoPricingParams& p, const char** why);
+// YieldCurve::from_points checks of a tenor list (false with *why set).
+bool validate_curve(const MangoTenorPoint* pts, uint64_t n, const char** why);
+// get_zero_rate (option_spec.hpp:99-112) of a valid curve: -ln D(T) / T, or
+// the forward rate at 0 for T <= 0 (YieldCurve::zero_rate, yield_curve.hpp:126-130).
+double curve_zero_rate(const MangoTenorPoint* pts, uint64_t n, double T);
 // Two curves equal within 1e-12 per entry after sorting (YieldCurve::operator==).
 bool same_curve(const MangoPricingParams& a, const MangoPricingParams& b);
 // PDEGridConfig{spec, n_time, dividend dates} resolved per contract:

@@ -37,6 +37,7 @@
 #include "bspline_basis.hpp"
 #include "mango_grid.hpp"
 #include "mango_host.hpp"
+#include "mango_events.hpp"
 #include "price_table.hpp"
 
 namespace mango_amd {
@@ -585,10 +586,10 @@ IvConfig to_iv_config(const MangoInterpSolverConfig* c) {
 // with *why set when the C struct cannot become a query.
 bool to_query(const MangoIvQuery& q, IvQueryRec& rec, const char** why) {
     if (q.option_type != MANGO_CALL && q.option_type != MANGO_PUT) { *why = "invalid option_type"; return false; }
-    if (q.n_tenor_points != 0) { *why = "yield curves are not supported by the GPU interpolated IV solver"; return false; }
+    if (!validate_curve(q.tenor_points, q.n_tenor_points, why)) return false;
     if (q.dividends == nullptr && q.n_dividends > 0) { *why = "dividends is null but n_dividends > 0"; return false; }
     if (!std::isfinite(q.dividend_yield)) { *why = "dividend_yield must be finite"; return false; }
-    if (!std::isfinite(q.rate_const)) { *why = "rate must be finite"; return false; }
+    if (q.n_tenor_points == 0 && !std::isfinite(q.rate_const)) { *why = "rate must be finite"; return false; }
     for (uint64_t i = 0; i < q.n_dividends; ++i) {
         const double ct = q.dividends[i].calendar_time, amt = q.dividends[i].amount;
         if (!std::isfinite(ct) || ct < 0.0 || ct > q.maturity || !std::isfinite(amt) || amt < 0.0) {
@@ -596,7 +597,12 @@ bool to_query(const MangoIvQuery& q, IvQueryRec& rec, const char** why) {
             return false;
         }
     }
-    rec = IvQueryRec{q.spot, q.strike, q.maturity, q.rate_const, q.dividend_yield, q.market_price, q.option_type, 0};
+    // a yield curve is collapsed to its zero rate at the maturity, flagged in
+    // `reserved` (interpolated_iv_solver.hpp:455-470, 536)
+    const bool curve = q.n_tenor_points > 0;
+    const double rate = curve ? curve_zero_rate(q.tenor_points, q.n_tenor_points, q.maturity) : q.rate_const;
+    rec = IvQueryRec{q.spot, q.strike, q.maturity, rate, q.dividend_yield, q.market_price, q.option_type,
+                     curve ? 1 : 0};
     return true;
 }
 
@@ -681,10 +687,15 @@ MangoStatus run_price(const TableData& t, const std::vector<double>& rows, std::
     return MANGO_OK;
 }
 
+// (S, K, tau, sigma, r) of a table query; a yield curve enters as its zero
+// rate at the maturity (get_zero_rate in eep_layer.hpp:39, 65).
 bool price_row(const MangoPricingParams* p, double* row) {
-    if (!p || p->n_tenor_points != 0 || (p->dividends == nullptr && p->n_dividends > 0)) return false;
+    if (!p || (p->dividends == nullptr && p->n_dividends > 0)) return false;
     if (p->option_type != MANGO_CALL && p->option_type != MANGO_PUT) return false;
-    row[0] = p->spot; row[1] = p->strike; row[2] = p->maturity; row[3] = p->volatility; row[4] = p->rate_const;
+    const char* why = nullptr;
+    if (!validate_curve(p->tenor_points, p->n_tenor_points, &why)) return false;
+    row[0] = p->spot; row[1] = p->strike; row[2] = p->maturity; row[3] = p->volatility;
+    row[4] = p->n_tenor_points > 0 ? curve_zero_rate(p->tenor_points, p->n_tenor_points, p->maturity) : p->rate_const;
     return true;
 }
 
@@ -791,13 +802,15 @@ MangoStatus mango_make_interp_iv_solver(const MangoIvFactoryConfig* cfg, MangoIn
 MangoStatus mango_price_table_validate(const MangoPriceTable* t, const MangoPricingParams* p, MangoError* err) {
     if (!t || !p) return fail(err, MANGO_ERR_VALIDATION, "null table or params");
     const char* why = nullptr;
-    const MangoStatus base = validate_params(*p, &why);
+    const MangoStatus base = validate_params(*p, &why, true);
     if (base != MANGO_OK) return fail(err, MANGO_ERR_VALIDATION, why ? why : "invalid pricing parameters");
     const TableData& d = *t->t;
     if (p->option_type != d.type) return fail(err, MANGO_ERR_VALIDATION, "option type does not match the table");
     if (std::abs(p->dividend_yield - d.q) > 1e-10) return fail(err, MANGO_ERR_VALIDATION, "dividend yield does not match the table");
     const double lm = std::log(p->spot / p->strike);
-    const double v[4] = {lm, p->maturity, p->volatility, p->rate_const};
+    const double rate = p->n_tenor_points > 0 ? curve_zero_rate(p->tenor_points, p->n_tenor_points, p->maturity)
+                                              : p->rate_const;
+    const double v[4] = {lm, p->maturity, p->volatility, rate};
     static const char* names[4] = {"log-moneyness", "maturity", "volatility", "rate"};
     for (int a = 0; a < kTableDims; ++a) {
         const auto& g = d.axes[static_cast<size_t>(a)];
@@ -933,6 +946,7 @@ MangoStatus mango_interp_iv_solve_batch(const MangoInterpIvSolver* s, const Mang
                 slot.success.implied_vol = res[k].implied_vol;
                 slot.success.iterations = static_cast<uint64_t>(res[k].iterations);
                 slot.success.final_error = res[k].final_error;
+                slot.success.used_rate_approximation = recs[k].reserved;
             }
         }
         if (out_failed_count) {
@@ -960,6 +974,7 @@ MangoStatus mango_interp_iv_solve(const MangoInterpIvSolver* s, const MangoIvQue
     out->implied_vol = res[0].implied_vol;
     out->iterations = static_cast<uint64_t>(res[0].iterations);
     out->final_error = res[0].final_error;
+    out->used_rate_approximation = r.reserved;
     return MANGO_OK;
 }
 

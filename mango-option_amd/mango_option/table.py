@@ -115,10 +115,26 @@ def _darr(v):
     return a, a.ctypes.data_as(_dp)
 
 
+def _set_rate(c, rate) -> None:
+    """rate_const, or the tenor points of a YieldCurve (the array is kept on
+    the struct object; the library collapses the curve to its zero rate at
+    the maturity, like the reference's surfaces)."""
+    from . import YieldCurve
+    if isinstance(rate, YieldCurve):
+        pts = (_abi.MangoTenorPoint * max(1, len(rate.points)))(*[_abi.MangoTenorPoint(t, ld) for t, ld in rate.points])
+        c._tenors = pts
+        c.rate_const = 0.0
+        c.tenor_points = C.cast(pts, C.POINTER(_abi.MangoTenorPoint))
+        c.n_tenor_points = len(rate.points)
+    else:
+        c.rate_const = rate
+
+
 def _c_query(q: IVQuery) -> MangoIvQuery:
     c = MangoIvQuery()
     c.spot, c.strike, c.maturity = q.spot, q.strike, q.maturity
-    c.dividend_yield, c.market_price, c.rate_const = q.dividend_yield, q.market_price, q.rate
+    c.dividend_yield, c.market_price = q.dividend_yield, q.market_price
+    _set_rate(c, q.rate)
     c.option_type = int(q.option_type)
     return c
 
@@ -126,7 +142,8 @@ def _c_query(q: IVQuery) -> MangoIvQuery:
 def _c_pricing(p) -> MangoPricingParams:
     c = MangoPricingParams()
     c.spot, c.strike, c.maturity = p.spot, p.strike, p.maturity
-    c.dividend_yield, c.volatility, c.rate_const = p.dividend_yield, p.volatility, p.rate
+    c.dividend_yield, c.volatility = p.dividend_yield, p.volatility
+    _set_rate(c, p.rate)
     c.option_type = int(p.option_type)
     return c
 
@@ -173,13 +190,15 @@ class InterpolatedIVSolver:
         err = MangoError()
         rc = self._lib.mango_interp_iv_solve(self._h, C.byref(_c_query(query)), C.byref(out), C.byref(err))
         if rc == MANGO_OK:
-            return True, IVSuccess(out.implied_vol, int(out.iterations), out.final_error), IVError()
+            return True, IVSuccess(out.implied_vol, int(out.iterations), out.final_error,
+                                   used_rate_approximation=bool(out.used_rate_approximation)), IVError()
         return False, IVSuccess(), IVError(code=rc, message=err.message.decode(errors="replace"))
 
     def solve_batch(self, queries):
         queries = list(queries)
         n = len(queries)
-        arr = (MangoIvQuery * max(1, n))(*[_c_query(q) for q in queries])
+        cq = [_c_query(q) for q in queries]  # alive until the call returns (tenor arrays)
+        arr = (MangoIvQuery * max(1, n))(*cq)
         slots = (MangoIvBatchSlot * max(1, n))()
         failed = C.c_uint64()
         err = MangoError()
@@ -190,7 +209,9 @@ class InterpolatedIVSolver:
             s = slots[i]
             if s.status == MANGO_OK:
                 res.append((True, IVSuccess(s.success.implied_vol, int(s.success.iterations),
-                                            s.success.final_error), IVError()))
+                                            s.success.final_error,
+                                            used_rate_approximation=bool(s.success.used_rate_approximation)),
+                             IVError()))
             else:
                 res.append((False, IVSuccess(), IVError(code=s.status)))
         return res, int(failed.value)

@@ -243,3 +243,27 @@ def test_c4_table_greeks(c4):
                           option_type=M.OptionType.PUT, volatility=0.3)
     with pytest.raises(ValueError):
         t.delta(out)
+
+
+def test_yield_curve_queries_use_the_zero_rate(c4):
+    # a curve enters the surface as its zero rate -ln D(T)/T
+    # (interpolated_iv_solver.hpp:455-470; eep_layer.hpp:39): the same
+    # result as the constant zero rate, and the success flags the collapse
+    t, _ = c4
+    solver = t.make_iv_solver()
+    curve = M.YieldCurve.from_points([(0.0, 0.0), (0.4, -0.008), (1.0, -0.032), (3.0, -0.14)])
+    for T in (0.3, 0.7, 1.2):
+        rz = -curve.log_discount(T) / T
+        q_c = M.IVQuery(100.0, 100.0, T, curve, W.C4_DIVIDEND, M.OptionType.PUT, 6.0)
+        q_r = M.IVQuery(100.0, 100.0, T, rz, W.C4_DIVIDEND, M.OptionType.PUT, 6.0)
+        ok_c, s_c, _ = solver.solve(q_c)
+        ok_r, s_r, _ = solver.solve(q_r)
+        assert ok_c and ok_r and s_c.used_rate_approximation and not s_r.used_rate_approximation
+        assert abs(s_c.implied_vol - s_r.implied_vol) <= 1e-12
+        (ok_b, s_b, _), = solver.solve_batch([q_c])[0]
+        assert ok_b and s_b.used_rate_approximation and abs(s_b.implied_vol - s_c.implied_vol) <= 1e-12
+        pc = M.PricingParams(spot=100.0, strike=100.0, maturity=T, rate=curve, dividend_yield=W.C4_DIVIDEND,
+                             option_type=M.OptionType.PUT, volatility=0.25)
+        pr = M.PricingParams(spot=100.0, strike=100.0, maturity=T, rate=rz, dividend_yield=W.C4_DIVIDEND,
+                             option_type=M.OptionType.PUT, volatility=0.25)
+        assert abs(t.price(pc) - t.price(pr)) <= 1e-12
