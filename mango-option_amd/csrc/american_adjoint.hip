@@ -111,7 +111,7 @@ __device__ __forceinline__ void adjoint_one(const PdeJob* __restrict__ jobs, int
     double* tr = scr + static_cast<size_t>(ck_cap) * NMAX;
 
     // ---- 1. forward with checkpoints ----
-    StageCache<M> c0, c1;
+    StageCache<K, M> c0, c1;
     c0.invalidate();
     c1.invalidate();
     double u[M], mid[M], up[M];

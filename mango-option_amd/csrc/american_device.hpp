@@ -396,6 +396,120 @@ __device__ __forceinline__ Aff scan_suffix(Aff c, int j) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Factorization-only chains of the two stage scans. In the d' suffix scan
+// the B component is a product of nlo/denom terms and in the monotone back
+// substitution the a component a product of -c' terms: neither depends on
+// the right-hand side, so their per-level values are computed once per
+// factorization (by the same products, in the same order, as the full scans)
+// and each stage scans only A, resp. (b, lo).
+// ---------------------------------------------------------------------------
+template <int K>
+struct SuffixCache {
+    double b[4];   // lane's B entering in-row level 0..3
+    double b4;     // lane's B after the in-row levels (K > 16)
+    double r1, r2; // B of the aggregates of rows 1 and 2 (K == 64)
+};
+
+template <int K>
+__device__ __forceinline__ void suffix_cache_build(SuffixCache<K>& s, double B) {
+    if constexpr (K == 16) {
+#define MANGO_STEP0(L, D) { s.b[L] = B; B = B * dpp0<kRowShl + D>(B); }
+        MANGO_STEP0(0, 1) MANGO_STEP0(1, 2) MANGO_STEP0(2, 4) MANGO_STEP0(3, 8)
+#undef MANGO_STEP0
+    } else {
+#define MANGO_STEP(L, D) { s.b[L] = B; B = B * dpp<kRowShl + D>(1.0, B); }
+        MANGO_STEP(0, 1) MANGO_STEP(1, 2) MANGO_STEP(2, 4) MANGO_STEP(3, 8)
+#undef MANGO_STEP
+        s.b4 = B;
+        if constexpr (K == 64) {
+            s.r1 = readlane_d(B, 16);
+            s.r2 = readlane_d(B, 32);
+        }
+    }
+}
+
+// A of scan_suffix given the lane's own A and the factorization's chain.
+template <int K>
+__device__ __forceinline__ double scan_suffix_A(double A, const SuffixCache<K>& s, int j) {
+    if constexpr (K == 16) {
+#define MANGO_STEP0(L, D) { A = fma(s.b[L], dpp0<kRowShl + D>(A), A); }
+        MANGO_STEP0(0, 1) MANGO_STEP0(1, 2) MANGO_STEP0(2, 4) MANGO_STEP0(3, 8)
+#undef MANGO_STEP0
+        return A;
+    } else {
+#define MANGO_STEP(L, D) { A = fma(s.b[L], dpp<kRowShl + D>(0.0, A), A); }
+        MANGO_STEP(0, 1) MANGO_STEP(1, 2) MANGO_STEP(2, 4) MANGO_STEP(3, 8)
+#undef MANGO_STEP
+        const int wl = __lane_id();
+        if constexpr (K == 32) {
+            const double a1 = readlane_d(A, 16);
+            const double a3 = readlane_d(A, 48);
+            const double nxt = (wl < 32) ? a1 : a3;
+            if ((j & 16) == 0) A = fma(s.b4, nxt, A);
+        } else {
+            const double a1 = readlane_d(A, 16);
+            const double a2 = readlane_d(A, 32);
+            const double a3 = readlane_d(A, 48);
+            const double s2 = fma(s.r2, a3, a2);
+            const double s1 = fma(s.r1, s2, a1);
+            const int row = j >> 4;
+            if (row < 3) A = fma(s.b4, row == 0 ? s1 : (row == 1 ? s2 : a3), A);
+        }
+        return A;
+    }
+}
+
+template <int K>
+struct PrefixCache {
+    double a[6];  // lane's a entering each level of the monotone prefix scan
+};
+
+template <int K>
+__device__ __forceinline__ void prefix_cache_build(PrefixCache<K>& c, double a) {
+    if constexpr (K == 16) {
+#define MANGO_STEP0(L, D) { c.a[L] = a; a = a * dpp0<kRowShr + D>(a); }
+        MANGO_STEP0(0, 1) MANGO_STEP0(1, 2) MANGO_STEP0(2, 4) MANGO_STEP0(3, 8)
+#undef MANGO_STEP0
+    } else {
+#define MANGO_STEP(L, CTRL, ROWS) { c.a[L] = a; a = a * dpp<CTRL, ROWS>(1.0, a); }
+        MANGO_STEP(0, kRowShr + 1, 0xf) MANGO_STEP(1, kRowShr + 2, 0xf) MANGO_STEP(2, kRowShr + 4, 0xf)
+        MANGO_STEP(3, kRowShr + 8, 0xf)
+        if constexpr (K > 16) MANGO_STEP(4, kRowBcast15, 0xa)
+        if constexpr (K > 32) MANGO_STEP(5, kRowBcast31, 0xc)
+#undef MANGO_STEP
+    }
+}
+
+// (b, lo) of scan_prefix<K>(Mono) given the lane's own (b, lo) and the chain.
+template <int K>
+__device__ __forceinline__ void scan_prefix_mono(double& b, double& lo, const PrefixCache<K>& c) {
+    if constexpr (K == 16) {
+#define MANGO_STEP0(L, D)                                                         \
+    {                                                                             \
+        const double qb = dpp0<kRowShr + D>(b), ql = dpp0<kRowShr + D>(lo);       \
+        const double nl = fmax(fma(c.a[L], ql, b), lo);                          \
+        b = fma(c.a[L], qb, b);                                                   \
+        lo = nl;                                                                  \
+    }
+        MANGO_STEP0(0, 1) MANGO_STEP0(1, 2) MANGO_STEP0(2, 4) MANGO_STEP0(3, 8)
+#undef MANGO_STEP0
+    } else {
+#define MANGO_STEP(L, CTRL, ROWS)                                                 \
+    {                                                                             \
+        const double qb = dpp<CTRL, ROWS>(0.0, b), ql = dpp<CTRL, ROWS>(-kBig, lo); \
+        const double nl = fmax(fma(c.a[L], ql, b), lo);                          \
+        b = fma(c.a[L], qb, b);                                                   \
+        lo = nl;                                                                  \
+    }
+        MANGO_STEP(0, kRowShr + 1, 0xf) MANGO_STEP(1, kRowShr + 2, 0xf) MANGO_STEP(2, kRowShr + 4, 0xf)
+        MANGO_STEP(3, kRowShr + 8, 0xf)
+        if constexpr (K > 16) MANGO_STEP(4, kRowBcast15, 0xa)
+        if constexpr (K > 32) MANGO_STEP(5, kRowBcast31, 0xc)
+#undef MANGO_STEP
+    }
+}
+
 template <int K>
 __device__ __forceinline__ uint64_t group_lanes() {
     if constexpr (K == 64) return ~0ull;
@@ -447,9 +561,11 @@ __device__ __forceinline__ double group_sum(double v) {
 
 // c' and 1/denom of one stage weight for the current lock pattern (the
 // factorization depends only on the bands, i.e. on weight and locks).
-template <int M>
+template <int K, int M>
 struct StageCache {
     double cp[M], iv[M];
+    SuffixCache<K> sc;  // factorization-only chains of the d' and
+    PrefixCache<K> pc;  // monotone back-substitution scans
     uint32_t mask;
     bool sing, mono;
     __device__ __forceinline__ void invalidate() {
@@ -664,7 +780,7 @@ struct OptionPde {
     // Stage matrix rows (Dirichlet rows, deep-ITM locks), c'/1/denom from the
     // cache (refreshed when the lock pattern changed) and d'.
     __device__ __forceinline__ bool eliminate(double gr, double w, const double (&rhs)[M], const double (&guess)[M],
-                                              StageCache<M>& c, double (&nlo)[M], double (&dp)[M], uint32_t& mask) const {
+                                              StageCache<K, M>& c, double (&nlo)[M], double (&dp)[M], uint32_t& mask) const {
         double rb[M];
         mask = 0;
 #pragma unroll
@@ -725,17 +841,24 @@ struct OptionPde {
             for (int k = 0; k < M; ++k) pos = pos || (c.cp[k] > 0.0);
             c.mono = (__ballot(pos) & gmask) == 0;
             c.mask = mask;
+            double B = 1.0, a = 1.0;
+#pragma unroll
+            for (int k = 0; k < M; ++k) B = (nlo[k] * c.iv[k]) * B;
+#pragma unroll
+            for (int k = M - 1; k >= 0; --k) a = -c.cp[k] * a;
+            suffix_cache_build<K>(c.sc, B);
+            prefix_cache_build<K>(c.pc, a);
         }
         if (c.sing) return false;
-        Aff f{0.0, 1.0};
+        double fA = 0.0;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             const double al = rb[k] * c.iv[k];
             const double be = nlo[k] * c.iv[k];
-            f = Aff{fma(be, f.A, al), be * f.B};
+            fA = fma(be, fA, al);
         }
-        f = scan_suffix<K>(f, lane);
-        double dprev = dpp<kWaveShl1>(0.0, f.A);
+        fA = scan_suffix_A<K>(fA, c.sc, lane);
+        double dprev = dpp<kWaveShl1>(0.0, fA);
         if (lane == K - 1) dprev = 0.0;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
@@ -749,18 +872,23 @@ struct OptionPde {
     // pde_solver.hpp:525-729): u = projected solve, then the boundary values.
     // gr: the right boundary value at the stage time.
     __device__ __forceinline__ bool stage(double gr, double w, const double (&rhs)[M], const double (&guess)[M],
-                                          StageCache<M>& c, double (&u)[M]) const {
+                                          StageCache<K, M>& c, double (&u)[M]) const {
         double nlo[M], dp[M];
         uint32_t mask;
         if (!eliminate(gr, w, rhs, guess, c, nlo, dp, mask)) return false;
         double v;
         if (c.mono) {
-            Mono F{1.0, 0.0, -kBig};
+            double fb = 0.0, flo = -kBig;
 #pragma unroll
-            for (int k = M - 1; k >= 0; --k) F = compose_row_mono(-c.cp[k], dp[k], psi[k], F);
-            F = scan_prefix<K>(F);
-            const double eb = dpp<kWaveShr1>(0.0, F.b);
-            const double elo = dpp<kWaveShr1>(-kBig, F.lo);
+            for (int k = M - 1; k >= 0; --k) {
+                const double m = -c.cp[k];
+                const double nl = fmax(fma(m, flo, dp[k]), psi[k]);
+                fb = fma(m, fb, dp[k]);
+                flo = nl;
+            }
+            scan_prefix_mono<K>(fb, flo, c.pc);
+            const double eb = dpp<kWaveShr1>(0.0, fb);
+            const double elo = dpp<kWaveShr1>(-kBig, flo);
             v = (lane == 0) ? 0.0 : fmax(eb, elo);
         } else {
             Clamp F = clamp_identity();
@@ -801,7 +929,7 @@ struct OptionPde {
     // in place; mid receives the intermediate state (half step / u*).
     // c0 serves the Euler half steps (then is invalidated) and stage 1, c1 stage 2.
     __device__ __forceinline__ bool advance(int64_t step, double dt, double (&u)[M], double (&mid)[M],
-                                            StageCache<M>& c0, StageCache<M>& c1) const {
+                                            StageCache<K, M>& c0, StageCache<K, M>& c1) const {
         const TrConstants tr;
         double un[M], rhs[M];
 #pragma unroll
@@ -842,7 +970,7 @@ struct OptionPde {
     // the kind's current factorization c: db_i = hB_i db_{i+1} + H_i (node-
     // decreasing scan), then gd += -w db, gl += w db c'_{i-1},
     // gu += -w nlo_{i+1} db_{i+1} / den_i on the free interior rows.
-    __device__ __forceinline__ void flush(const StageCache<M>& c, DenAcc<M>& a, double (&gl)[M], double (&gd)[M],
+    __device__ __forceinline__ void flush(const StageCache<K, M>& c, DenAcc<M>& a, double (&gl)[M], double (&gd)[M],
                                           double (&gu)[M]) const {
         if (!a.active) return;
         const double w = a.w;
@@ -897,7 +1025,7 @@ struct OptionPde {
     //     cbar'_i = -g_i u_{i+1} + nlo_{i+1} denbar_{i+1}             (downward)
     // t and disc = e^{-r t}: the stage time and its discount factor.
     __device__ __forceinline__ bool stage_reverse(double t, double disc, double w, const double (&rhs)[M],
-                                                  const double (&guess)[M], StageCache<M>& c, DenAcc<M>& acc,
+                                                  const double (&guess)[M], StageCache<K, M>& c, DenAcc<M>& acc,
                                                   const double (&uo)[M], const double (&ubar)[M], double (&rbf)[M],
                                                   double (&gl)[M], double (&gd)[M], double (&gu)[M],
                                                   double& rbc) const {

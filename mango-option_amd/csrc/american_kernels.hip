@@ -167,8 +167,11 @@ __device__ __forceinline__ void solve_one(const PdeJob* __restrict__ jobs, int j
     // position of node n-1 in this lane (-1 if elsewhere); node 0 is lane K-1, k = 0
     const int k_right = (n - 1 >= base && n - 1 < base + M) ? n - 1 - base : -1;
 
-    // c' and 1/denom per stage kind (0: Euler half step, then stage 1; 1: stage 2)
+    // c' and 1/denom per stage kind (0: Euler half step, then stage 1; 1: stage 2),
+    // with the factorization-only chains of the d' and back-substitution scans
     double cp0[M], iv0[M], cp1[M], iv1[M];
+    SuffixCache<K> sc0, sc1;
+    PrefixCache<K> pc0, pc1;
     uint32_t cmask0 = ~0u, cmask1 = ~0u;
     bool csing0 = false, csing1 = false;
     bool cmono0 = false, cmono1 = false;
@@ -177,7 +180,8 @@ __device__ __forceinline__ void solve_one(const PdeJob* __restrict__ jobs, int j
     // pde_solver.hpp:525-729): bands, Dirichlet rows, deep-ITM locks,
     // projected Thomas, boundary values. rhs/guess in, u out.
     auto stage = [&](double gr, double w, const double (&rhs)[M], const double (&guess)[M], double (&cp)[M],
-                     double (&iv)[M], uint32_t& cmask, bool& csing, bool& cmono) -> bool {
+                     double (&iv)[M], SuffixCache<K>& sc, PrefixCache<K>& pc, uint32_t& cmask, bool& csing,
+                     bool& cmono) -> bool {
         double nlo[M], rb[M];
         uint32_t mask = 0;
 #pragma unroll
@@ -243,18 +247,27 @@ __device__ __forceinline__ void solve_one(const PdeJob* __restrict__ jobs, int j
             for (int k = 0; k < M; ++k) pos = pos || (cp[k] > 0.0);
             cmono = (__ballot(pos) & gmask) == 0;
             cmask = mask;
+            // factorization-only chains of the two scans (same products as
+            // the full scans compute per stage)
+            double B = 1.0, a = 1.0;
+#pragma unroll
+            for (int k = 0; k < M; ++k) B = (nlo[k] * iv[k]) * B;
+#pragma unroll
+            for (int k = M - 1; k >= 0; --k) a = -cp[k] * a;
+            suffix_cache_build<K>(sc, B);
+            prefix_cache_build<K>(pc, a);
         }
         if (csing) return false;
         // --- d' = (rhs - l d'_prev) / denom: suffix scan of affine maps ---
-        Aff f{0.0, 1.0};
+        double fA = 0.0;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             const double al = rb[k] * iv[k];
             const double be = nlo[k] * iv[k];
-            f = Aff{fma(be, f.A, al), be * f.B};
+            fA = fma(be, fA, al);
         }
-        f = scan_suffix<K>(f, lane);
-        double dprev = dpp<kWaveShl1>(0.0, f.A);
+        fA = scan_suffix_A<K>(fA, sc, lane);
+        double dprev = dpp<kWaveShl1>(0.0, fA);
         if (lane == K - 1) dprev = 0.0;
         double dp[M];
 #pragma unroll
@@ -265,12 +278,17 @@ __device__ __forceinline__ void solve_one(const PdeJob* __restrict__ jobs, int j
         // --- projected back substitution: prefix scan of clamped-affine maps ---
         double v;
         if (cmono) {
-            Mono F{1.0, 0.0, -kBig};
+            double fb = 0.0, flo = -kBig;
 #pragma unroll
-            for (int k = M - 1; k >= 0; --k) F = compose_row_mono(-cp[k], dp[k], psi[k], F);
-            F = scan_prefix<K>(F);
-            const double eb = dpp<kWaveShr1>(0.0, F.b);
-            const double elo = dpp<kWaveShr1>(-kBig, F.lo);
+            for (int k = M - 1; k >= 0; --k) {
+                const double m = -cp[k];
+                const double nl = fmax(fma(m, flo, dp[k]), psi[k]);
+                fb = fma(m, fb, dp[k]);
+                flo = nl;
+            }
+            scan_prefix_mono<K>(fb, flo, pc);
+            const double eb = dpp<kWaveShr1>(0.0, fb);
+            const double elo = dpp<kWaveShr1>(-kBig, flo);
             v = (lane == 0) ? 0.0 : fmax(eb, elo);
         } else {
             Clamp F = clamp_identity();
@@ -304,11 +322,11 @@ __device__ __forceinline__ void solve_one(const PdeJob* __restrict__ jobs, int j
         const double t_mid = t0 + half_dt;
 #pragma unroll
         for (int k = 0; k < M; ++k) { up[k] = u[k]; rhs[k] = up[k]; }
-        ok = stage(g_right(t_mid), half_dt, rhs, up, cp0, iv0, cmask0, csing0, cmono0);
+        ok = stage(g_right(t_mid), half_dt, rhs, up, cp0, iv0, sc0, pc0, cmask0, csing0, cmono0);
         if (ok) {
 #pragma unroll
             for (int k = 0; k < M; ++k) { up[k] = u[k]; rhs[k] = up[k]; }
-            ok = stage(g_right(t_next), half_dt, rhs, up, cp0, iv0, cmask0, csing0, cmono0);
+            ok = stage(g_right(t_next), half_dt, rhs, up, cp0, iv0, sc0, pc0, cmask0, csing0, cmono0);
         }
         cmask0 = ~0u;  // the Euler weight differs from stage 1's
     }
@@ -346,14 +364,14 @@ __device__ __forceinline__ void solve_one(const PdeJob* __restrict__ jobs, int j
                     rhs[k] = fma(w1, lu, up[k]);  // jl = jd = ju = 0 off the interior: L = 0
                 }
             }
-            ok = stage(g_right_d(disc * e_g), w1, rhs, up, cp0, iv0, cmask0, csing0, cmono0);
+            ok = stage(g_right_d(disc * e_g), w1, rhs, up, cp0, iv0, sc0, pc0, cmask0, csing0, cmono0);
             if (!ok) break;
             // stage 2: rhs = alpha u* + beta u^n, guess u* (pde_solver.hpp:370-395)
             double ustar[M];
 #pragma unroll
             for (int k = 0; k < M; ++k) { ustar[k] = u[k]; rhs[k] = fma(alpha, u[k], beta * up[k]); }
             disc *= e_dt;
-            ok = stage(g_right_d(disc), w2, rhs, ustar, cp1, iv1, cmask1, csing1, cmono1);
+            ok = stage(g_right_d(disc), w2, rhs, ustar, cp1, iv1, sc1, pc1, cmask1, csing1, cmono1);
             if (!ok) break;
             t = t_next;
         }

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kSolveBlock) void table_snapshot_kernel(const PdeJo
             if (i < n) xr[i] = s_x[slot][i];
         }
     }
-    StageCache<M> c0, c1;
+    StageCache<K, M> c0, c1;
     c0.invalidate();
     c1.invalidate();
     double u[M], mid[M];
