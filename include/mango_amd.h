@@ -442,6 +442,24 @@ MangoStatus mango_interp_iv_solve_batch(const MangoInterpIvSolver* s, const Mang
                                         MangoIvBatchSlot* out_slots, uint64_t* out_failed_count, MangoError* err);
 void mango_interp_iv_solver_free(MangoInterpIvSolver* s);
 
+/* ---- FDM implied volatility: src/ffi/mango_c_api.h:69, 152-155 ----------
+ * IVSolver (src/option/iv_solver.cpp:20-209): Brent's method on [0.01, 1.5 |
+ * 2 | 3] (upper bound by the time-value ratio) where every objective value is
+ * a full American PDE solve on the estimated grid (GridAccuracyParams{}),
+ * with the query's yield curve and discrete dividends. max_iter 0 and
+ * brent_tol_abs 0 mean the defaults (100, 1e-6); a negative max_iter is a
+ * validation error. Brent runs on the host; each round of objective values
+ * of a batch is one GPU launch. */
+typedef struct { double brent_tol_abs; int32_t max_iter; } MangoIvConfig;
+
+/* Replaces mango_c_api.h:152-155. */
+MangoStatus mango_solve_iv(const MangoIvQuery* query, const MangoIvConfig* config, MangoIvSuccess* out_success,
+                           MangoError* out_err);
+/* IVSolver::solve_batch (iv_solver.cpp:211-240; no C entry in the
+ * reference): per-slot status and success, failed count. */
+MangoStatus mango_solve_iv_batch(const MangoIvQuery* queries, uint64_t n, const MangoIvConfig* config,
+                                 MangoIvBatchSlot* out_slots, uint64_t* out_failed_count, MangoError* err);
+
 /* ---- price table (additions) --------------------------------------------
  * PriceTableBuilder::from_vectors(...) + build(axes, EEP transform)
  * (src/option/table/bspline/bspline_builder.cpp:26-134, 544-594;
@@ -520,6 +538,8 @@ static_assert(sizeof(MangoIvFactoryConfig) == 144, "MangoIvFactoryConfig size");
 static_assert(offsetof(MangoIvFactoryConfig, solver_config) == 88, "fc solver_config off");
 static_assert(offsetof(MangoIvFactoryConfig, adaptive) == 128, "fc adaptive off");
 static_assert(sizeof(MangoIvBatchSlot) == 48, "MangoIvBatchSlot size");
+static_assert(sizeof(MangoIvConfig) == 16, "MangoIvConfig size");
+static_assert(offsetof(MangoIvConfig, max_iter) == 8, "max_iter off");
 static_assert(sizeof(MangoTableBuildInfo) == 56, "MangoTableBuildInfo size");
 static_assert(sizeof(MangoDeviceIvQuery) == 56, "MangoDeviceIvQuery size");
 static_assert(sizeof(MangoDeviceIvResult) == 24, "MangoDeviceIvResult size");

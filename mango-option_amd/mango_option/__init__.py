@@ -516,9 +516,10 @@ class BatchAmericanOptionSolver:
 
 
 from .table import (BSplineBackend, InterpolatedIVSolver, InterpolatedIVSolverConfig, IVError, IVGrid,  # noqa: E402
-                    IVQuery, IVSolverFactoryConfig, IVSuccess, PriceTable, PriceTableBuilder, PriceTableConfig,
-                    make_interpolated_iv_solver, make_price_table)
+                    IVQuery, IVSolver, IVSolverConfig, IVSolverFactoryConfig, IVSuccess, PriceTable,
+                    PriceTableBuilder, PriceTableConfig, RootFindingConfig, make_interpolated_iv_solver,
+                    make_price_table)
 
 __all__ += ["BSplineBackend", "InterpolatedIVSolver", "InterpolatedIVSolverConfig", "IVError", "IVGrid", "IVQuery",
-            "IVSolverFactoryConfig", "IVSuccess", "PriceTable", "PriceTableBuilder", "PriceTableConfig",
-            "make_interpolated_iv_solver", "make_price_table"]
+            "IVSolver", "IVSolverConfig", "IVSolverFactoryConfig", "IVSuccess", "PriceTable", "PriceTableBuilder",
+            "PriceTableConfig", "RootFindingConfig", "make_interpolated_iv_solver", "make_price_table"]

@@ -195,6 +195,10 @@ class MangoIvBatchSlot(C.Structure):
     _fields_ = [("status", C.c_int32), ("success", MangoIvSuccess)]
 
 
+class MangoIvConfig(C.Structure):
+    _fields_ = [("brent_tol_abs", C.c_double), ("max_iter", C.c_int32)]
+
+
 class MangoTableBuildInfo(C.Structure):
     _fields_ = [("n_pde_solves", C.c_uint64), ("failed_pde", C.c_uint64), ("repaired_full_slices", C.c_uint64),
                 ("repaired_partial_points", C.c_uint64), ("fallback_grid", C.c_int32), ("reserved", C.c_int32),
@@ -303,6 +307,9 @@ SIGNATURES = {
     "mango_interp_iv_solve_batch": (C.c_int32, [C.c_void_p, _P(MangoIvQuery), C.c_uint64, _P(MangoIvBatchSlot),
                                                 _P(C.c_uint64), _P(MangoError)]),
     "mango_interp_iv_solver_free": (None, [C.c_void_p]),
+    "mango_solve_iv": (C.c_int32, [_P(MangoIvQuery), _P(MangoIvConfig), _P(MangoIvSuccess), _P(MangoError)]),
+    "mango_solve_iv_batch": (C.c_int32, [_P(MangoIvQuery), C.c_uint64, _P(MangoIvConfig), _P(MangoIvBatchSlot),
+                                         _P(C.c_uint64), _P(MangoError)]),
     "mango_price_table_build": (C.c_int32, [_P(C.c_double), C.c_uint64, _P(C.c_double), C.c_uint64,
                                             _P(C.c_double), C.c_uint64, _P(C.c_double), C.c_uint64, C.c_double,
                                             C.c_double, C.c_int32, _P(MangoGridConfig), C.c_double,

@@ -85,6 +85,7 @@ class IVQuery:
     dividend_yield: float = 0.0
     option_type: int = MANGO_PUT
     market_price: float = 0.0
+    discrete_dividends: list = field(default_factory=list)  # honored by the FDM IVSolver
 
 
 @dataclass
@@ -432,3 +433,80 @@ class PriceTableBuilder:
             C.byref(info), C.byref(err))
         _raise(rc, err, "mango_price_table_build")
         return PriceTable(out.value, info)
+
+
+# ---------------------------------------------------------------------------
+# FDM implied volatility (IVSolver; Python names of
+# src/python/mango_bindings.cpp:420-521)
+# ---------------------------------------------------------------------------
+@dataclass
+class RootFindingConfig:
+    max_iter: int = 100
+    tolerance: float = 1e-6
+    jacobian_fd_epsilon: float = 1e-7
+    brent_tol_abs: float = 1e-6
+
+
+@dataclass
+class IVSolverConfig:
+    root_config: RootFindingConfig = field(default_factory=RootFindingConfig)
+    batch_parallel_threshold: int = 4
+
+
+class IVSolver:
+    """IVSolver (src/option/iv_solver.hpp:70-198): Brent's method over full
+    American PDE solves on the GPU (mango_solve_iv / mango_solve_iv_batch);
+    yield curves and discrete dividends of the query are priced exactly."""
+
+    def __init__(self, config: IVSolverConfig | None = None):
+        self.config = config if config is not None else IVSolverConfig()
+        self._lib = _abi.load_library()
+
+    def _cfg(self):
+        c = _abi.MangoIvConfig()
+        c.max_iter = int(self.config.root_config.max_iter)
+        c.brent_tol_abs = float(self.config.root_config.brent_tol_abs)
+        return c
+
+    @staticmethod
+    def _c_fdm_query(q: IVQuery):
+        c = _c_query(q)
+        divs = q.discrete_dividends or []
+        if divs:
+            arr = (_abi.MangoDividend * len(divs))(*[_abi.MangoDividend(d.calendar_time, d.amount) for d in divs])
+            c._divs = arr
+            c.dividends = C.cast(arr, C.POINTER(_abi.MangoDividend))
+            c.n_dividends = len(divs)
+        return c
+
+    def solve(self, query: IVQuery):
+        """Returns (success, IVSuccess, IVError) like the reference binding."""
+        out = _abi.MangoIvSuccess()
+        err = MangoError()
+        rc = self._lib.mango_solve_iv(C.byref(self._c_fdm_query(query)), C.byref(self._cfg()), C.byref(out),
+                                      C.byref(err))
+        if rc == MANGO_OK:
+            return True, IVSuccess(out.implied_vol, int(out.iterations), out.final_error), IVError()
+        return False, IVSuccess(), IVError(code=rc, message=err.message.decode(errors="replace"))
+
+    def solve_batch(self, queries):
+        """IVSolver::solve_batch: ([(success, IVSuccess, IVError)], failed_count)."""
+        queries = list(queries)
+        n = len(queries)
+        if n == 0:
+            return [], 0
+        cq = [self._c_fdm_query(q) for q in queries]
+        arr = (MangoIvQuery * n)(*cq)
+        slots = (MangoIvBatchSlot * n)()
+        failed = C.c_uint64()
+        err = MangoError()
+        rc = self._lib.mango_solve_iv_batch(arr, n, C.byref(self._cfg()), slots, C.byref(failed), C.byref(err))
+        _raise(rc, err, "mango_solve_iv_batch")
+        res = []
+        for s in slots:
+            if s.status == MANGO_OK:
+                res.append((True, IVSuccess(s.success.implied_vol, int(s.success.iterations), s.success.final_error),
+                            IVError()))
+            else:
+                res.append((False, IVSuccess(), IVError(code=s.status)))
+        return res, int(failed.value)

@@ -6,6 +6,7 @@
 #include "mango_oracle.hpp"
 #include "mango_adjoint.hpp"
 #include "mango_table.hpp"
+#include "mango_fdm_iv.hpp"
 
 #include "../include/mango_amd.h"
 
@@ -570,6 +571,18 @@ void oracle_dividend_shift(const double* x, double* u, uint64_t n, double d, dou
     std::vector<double> xv(x, x + n), uv(u, u + n);
     dividend_shift(xv, uv, d, fallback);
     std::copy(uv.begin(), uv.end(), u);
+}
+
+// IVSolver::solve (FDM implied volatility); p's volatility is ignored.
+int32_t oracle_fdm_iv(const MangoPricingParams* p, double market, uint64_t max_iter, double tol, double* iv,
+                      double* final_error, uint64_t* iterations) {
+    PricingParams pp;
+    if (!to_params_ex(*p, pp)) return MANGO_ERR_VALIDATION;
+    const FdmIvResult r = fdm_iv(pp, market, max_iter, tol);
+    *iv = r.implied_vol;
+    *final_error = r.final_error;
+    *iterations = r.iterations;
+    return r.status;
 }
 
 double oracle_bs_price(double S, double K, double tau, double sigma, double r, double q, int32_t type) {

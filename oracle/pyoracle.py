@@ -82,6 +82,8 @@ def load() -> C.CDLL:
         "oracle_time_points": (C.c_uint64, [C.c_double, C.c_double, C.c_double, _dp, C.c_uint64, _dp, C.c_uint64,
                                             _dp]),
         "oracle_dividend_shift": (None, [_dp, _dp, C.c_uint64, C.c_double, C.c_double]),
+        "oracle_fdm_iv": (C.c_int32, [P(MangoPricingParams), C.c_double, C.c_uint64, C.c_double, _dp, _dp,
+                                      P(C.c_uint64)]),
         "oracle_bs_price": (C.c_double, [C.c_double] * 6 + [C.c_int32]),
         "oracle_bs_vega": (C.c_double, [C.c_double] * 6),
     }
@@ -413,3 +415,10 @@ def dividend_shift(x, u, d, fallback):
     x = _a(x); u = _a(u).copy()
     load().oracle_dividend_shift(_p(x), _p(u), len(x), d, fallback)
     return u
+
+
+def fdm_iv(pp: Params, market: float, max_iter: int = 100, tol: float = 1e-6):
+    """IVSolver::solve restated (Brent over full PDE solves): (status, iv, final_error, iterations)."""
+    iv = C.c_double(); fe = C.c_double(); it = C.c_uint64()
+    st = load().oracle_fdm_iv(C.byref(pp.p), market, max_iter, tol, C.byref(iv), C.byref(fe), C.byref(it))
+    return st, iv.value, fe.value, it.value
