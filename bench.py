@@ -24,6 +24,8 @@ Also measured in the same run:
   (mango_price_table_build, mango_interp_iv_solve_device);
 - events: 16,384 puts with a yield curve and two cash dividends each through
   the host-buffer batch entry (mango_price_american_batch_ex, event kernel);
+- fdm_iv: 4,096 FDM implied-volatility queries (mango_solve_iv_batch: Brent
+  over full PDE solves, one batched launch per Brent round);
 - cpu_baseline: the oracle (CPU restatement of the reference's OpenMP batch
   path) on the host cores, which also checks a sample of the GPU results.
 """
@@ -354,7 +356,52 @@ def run_events(g: Gpu, args):
             "_arr": arr, "_keep": keep, "_cfg": cfg, "_greeks": greeks[:args.event_check].copy()}
 
 
-def cpu_baseline(chains, batched, seconds: float, adjoint=None, table=None, events=None):
+def run_fdm_iv(g: Gpu, args):
+    """FDM implied volatility (mango_solve_iv_batch): market prices are the
+    GPU's own prices of C1-distribution puts at their volatility on the
+    default grid, so every query has a root; Brent over full PDE solves."""
+    batch = g.W.c1_puts(args.fdm_iv_queries, seed=7)
+    n = len(batch)
+    params = (g.abi.MangoPricingParams * n)()
+    for i in range(n):
+        p = params[i]
+        p.spot = float(batch.spot[i]); p.strike = float(batch.strike[i]); p.maturity = float(batch.maturity[i])
+        p.dividend_yield = float(batch.dividend_yield[i]); p.volatility = float(batch.volatility[i])
+        p.rate_const = float(batch.rate[i]); p.option_type = int(batch.option_type[i])
+    out = (g.abi.MangoAmericanGreeks * n)()
+    st = (C.c_int32 * n)()
+    failed = C.c_uint64()
+    g.check(g.lib.mango_price_american_batch_ex(params, n, 0, 0, None, out, st, C.byref(failed), C.byref(g.err)))
+    prices = np.frombuffer(out, dtype=np.float64).reshape(n, 4)[:, 0].copy()
+    qs = (g.abi.MangoIvQuery * n)()
+    for i in range(n):
+        q = qs[i]
+        q.spot, q.strike, q.maturity = params[i].spot, params[i].strike, params[i].maturity
+        q.dividend_yield, q.rate_const, q.option_type = params[i].dividend_yield, params[i].rate_const, params[i].option_type
+        q.market_price = float(prices[i])
+    slots = (g.abi.MangoIvBatchSlot * n)()
+
+    def step():
+        g.check(g.lib.mango_solve_iv_batch(qs, n, None, slots, C.byref(failed), C.byref(g.err)))
+
+    el, _, _ = g.timed(step, args.fdm_iv_steps, 0, kernel_timing=False)
+    stat = np.array([s.status for s in slots])
+    iv = np.array([s.success.implied_vol for s in slots])
+    its = np.array([s.success.iterations for s in slots])
+    ok = stat == 0
+    return {"workload": f"{n} FDM implied-vol queries (Brent over full PDE solves, default-accuracy per-option "
+                        f"grids) on the GPU's own prices of plain-distribution puts",
+            "queries": n, "queries_per_s": n * args.fdm_iv_steps / el, "ms_per_step": 1e3 * el / args.fdm_iv_steps,
+            "converged": int(ok.sum()), "mean_brent_iterations": float(its[ok].mean()) if ok.any() else None,
+            "max_abs_iv_err_vs_true_sigma": float(np.max(np.abs(iv[ok] - batch.volatility[ok]))) if ok.any() else None,
+            "_qs": qs, "_iv": iv, "_stat": stat}
+
+
+def args_fdm_check(fdm) -> int:
+    return int(fdm.get("_check", 256))
+
+
+def cpu_baseline(chains, batched, seconds: float, adjoint=None, table=None, events=None, fdm=None):
     """The oracle (CPU restatement of the reference) pricing whole chains the
     reference's way: one solve_batch(group, use_shared_grid=true, grid) per
     (chain, type) with the chain's union grid; OpenMP across the regular
@@ -425,6 +472,26 @@ def cpu_baseline(chains, batched, seconds: float, adjoint=None, table=None, even
         out["events"] = {"value": k / ev_s, "unit": "options/s", "cores": threads,
                          "sample": f"the first {k} options of the event workload, oracle OpenMP batch",
                          "parity_max_abs_price_err": float(np.max(np.abs(gr[:, 0] - events["_greeks"][:, 0])))}
+    if fdm is not None:
+        from concurrent.futures import ThreadPoolExecutor
+        k = min(args_fdm_check(fdm), len(fdm["_iv"]))
+        qs = fdm["_qs"]
+
+        def one(i):
+            q = qs[i]
+            pp = O.Params(q.spot, q.strike, q.maturity, 0.2, q.rate_const, q.dividend_yield, q.option_type == 1)
+            return O.fdm_iv(pp, q.market_price)
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            ref = list(ex.map(one, range(k)))
+        fdm_s = time.perf_counter() - t0
+        rst = np.array([r[0] for r in ref])
+        riv = np.array([r[1] for r in ref])
+        ok = (rst == 0) & (fdm["_stat"][:k] == 0)
+        out["fdm_iv"] = {"value": k / fdm_s, "unit": "queries/s", "cores": threads,
+                         "sample": f"the first {k} FDM IV queries, oracle IVSolver, {threads} threads",
+                         "parity_status_mismatches": int(np.sum(rst != fdm["_stat"][:k])),
+                         "parity_max_abs_iv_err": float(np.max(np.abs(riv[ok] - fdm["_iv"][:k][ok]), initial=0.0))}
     if adjoint is not None:
         ref, _ = O.adjoint_batch(adjoint["_batch"], adjoint["_cfg"], n_threads=threads)
         sens = adjoint["_out"][:, 4:8]
@@ -454,6 +521,9 @@ def main():
     ap.add_argument("--event-options", type=int, default=16384)
     ap.add_argument("--event-steps", type=int, default=3)
     ap.add_argument("--event-check", type=int, default=512, help="event options checked against the oracle")
+    ap.add_argument("--fdm-iv-queries", type=int, default=4096)
+    ap.add_argument("--fdm-iv-steps", type=int, default=2)
+    ap.add_argument("--fdm-iv-check", type=int, default=256, help="FDM IV queries checked against the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", default="", help="force lanes,nodes_per_lane of the solve kernel at 101 points")
@@ -470,6 +540,9 @@ def main():
     adjoint = run_adjoint(g, args) if args.adjoint_options > 0 else None
     table = None if args.no_table else run_table(g, args)
     events = run_events(g, args) if args.event_options > 0 else None
+    fdm = run_fdm_iv(g, args) if args.fdm_iv_queries > 0 else None
+    if fdm is not None:
+        fdm["_check"] = args.fdm_iv_check
 
     workload = "chains_4096_101x498"
     flops_launch = chains["pde_solves"] * float(flops_per_option(N_SPACE, N_TIME))
@@ -506,8 +579,10 @@ def main():
         line["table"] = {k: v for k, v in table.items() if not k.startswith("_")}
     if events is not None:
         line["events"] = {k: v for k, v in events.items() if not k.startswith("_")}
+    if fdm is not None:
+        line["fdm_iv"] = {k: v for k, v in fdm.items() if not k.startswith("_")}
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(chains, batched, args.cpu_seconds, adjoint, table,
-                                                                 events)
+                                                                 events, fdm)
     print(json.dumps(line), flush=True)
 
 

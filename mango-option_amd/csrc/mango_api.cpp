@@ -289,6 +289,21 @@ PdeJob* device_job_scratch(size_t n, MangoError* err) {
 int timed_events(const SolveVariant& v, const PdeJob* d_jobs, int n_jobs, const double* d_pool, Greeks* d_greeks,
                  int32_t* d_status, const StateOut& state, hipStream_t stream);
 
+// Streams for the independent per-variant launches of a host-buffer batch
+// (default priority, non-blocking; created once, kept for the process).
+std::mutex g_fan_mu;
+std::vector<hipStream_t> g_fan_streams;
+bool fan_streams(size_t k, std::vector<hipStream_t>& out) {
+    std::lock_guard<std::mutex> lk(g_fan_mu);
+    while (g_fan_streams.size() < k) {
+        hipStream_t s;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return false;
+        g_fan_streams.push_back(s);
+    }
+    out.assign(g_fan_streams.begin(), g_fan_streams.begin() + static_cast<std::ptrdiff_t>(k));
+    return true;
+}
+
 // Event jobs (ext != 0) binned by variant: appended to `sorted` with their
 // (variant, start, count) ranges. False if a grid is too large.
 bool bin_event_jobs(const std::vector<PdeJob>& jobs, std::vector<PdeJob>& sorted,
@@ -377,12 +392,20 @@ MangoStatus run_jobs(std::vector<PdeJob>& jobs, size_t n_out, Greeks* greeks, in
         so.x = d_x.p; so.u = d_u.p; so.u_prev = d_up.p; so.c_u = d_cu.p; so.c_up = d_cup.p;
         so.n_space = d_ns.p; so.dt = d_dt.p; so.stride = stride;
     }
+    // the launches are independent (disjoint jobs and output slots): with more
+    // than one they run side by side on their own streams
+    std::vector<hipStream_t> fan;
+    const size_t n_launch = ranges.size() + ev_ranges.size();
+    const bool use_fan = n_launch > 1 && fan_streams(n_launch, fan);
+    size_t li = 0;
     for (const auto& [v, r] : ranges)
         ok = ok && launch_american_solve(v, d_jobs.p + r.first, static_cast<int>(r.second), d_greeks.p,
-                                         d_status.p, so, stream) == 0;
+                                         d_status.p, so, use_fan ? fan[li++] : stream) == 0;
     for (const auto& [m, r] : ev_ranges)
         ok = ok && timed_events(m, d_jobs.p + r.first, static_cast<int>(r.second), d_pool.p, d_greeks.p,
-                                d_status.p, so, stream) == 0;
+                                d_status.p, so, use_fan ? fan[li++] : stream) == 0;
+    if (use_fan)
+        for (hipStream_t fs : fan) ok = ok && hipStreamSynchronize(fs) == hipSuccess;
     ok = ok && hipStreamSynchronize(stream) == hipSuccess;
     // Only the slots the jobs wrote are copied back.
     std::vector<Greeks> hg(n_out);
