@@ -254,8 +254,10 @@ MangoStatus mango_price_american_queue_device(const MangoDeviceOption* d_opts, u
 /* Reverse-mode sensitivities of the American price on each option's frozen
  * grid (the grid the solve uses for `grid`), with the forward's exercise
  * sets frozen: dV/dsigma, dV/dr, dV/dq by one reverse sweep through the
- * TR-BDF2 stages (states checkpointed every ceil(sqrt(N)) steps and
- * recomputed per segment), dV/dK = (V - S delta)/K (strike-free PDE), plus
+ * TR-BDF2 stages (the forward's states kept for every step when that tape
+ * fits the tape budget below, else checkpointed every ceil(sqrt(N)) steps and
+ * recomputed per segment; the two agree to rounding),
+ * dV/dK = (V - S delta)/K (strike-free PDE), plus
  * the forward's value, delta, gamma, theta. The reference has no adjoint; the
  * survey's batch adjoint export ("mango_price_american_batch_adjoint") is this
  * one, and its forward is solve_american_option on that grid
@@ -275,6 +277,12 @@ MangoStatus mango_price_american_batch_adjoint(const MangoPricingParams* params,
 MangoStatus mango_price_american_adjoint_device(const MangoDeviceOption* d_opts, uint64_t n,
                                                 const MangoGridConfig* grid, MangoAdjointGreeks* d_out,
                                                 int32_t* d_status, void* stream, MangoError* err);
+/* Device memory the adjoint may hold for whole-step tapes (two rows of the
+ * grid per time step per resident option group; 16,384 options at 101 x 498
+ * take about 3.7 GB); above it the adjoint checkpoints. 0 restores the
+ * default of 8 GiB. Returns the previous budget. No counterpart in the
+ * reference. */
+uint64_t mango_adjoint_tape_budget(uint64_t bytes);
 
 /* Per-option grid size the device entry points use for `grid` (host
  * memory in and out): n_space[i] and n_time[i] as estimate_pde_grid

@@ -8,12 +8,15 @@
 // pde_solver.hpp:96-729 on the grid of src/option/grid_spec_types.cpp:37-118.
 //
 // One option per group of K lanes, in the solve kernel's register layout.
-//   1. forward march, storing the state every S = ceil(sqrt(N)) steps
-//      (checkpoints, group scratch in HBM); Greeks at spot as in the solve;
+//   1. forward march. When the group's scratch holds the whole tape (u^n and
+//      the stage-1 state of every step: N <= seg_cap) the forward writes it
+//      and nothing is recomputed; otherwise it stores the state every
+//      S = ceil(sqrt(N)) steps (checkpoints, group scratch in HBM). Greeks at
+//      spot as in the solve;
 //   2. seed: dV/du^N = K ds(ln(S/K))/du of the natural spline (lane 0, LDS);
-//   3. segments from last to first: recompute the segment from its
-//      checkpoint storing u^n and the stage-1 state per step (group scratch),
-//      then reverse its steps: stage 2, then stage 1 (Rannacher's two Euler
+//   3. segments from last to first: (checkpointed jobs) recompute the segment
+//      from its checkpoint storing u^n and the stage-1 state per step, then
+//      reverse its steps: stage 2, then stage 1 (Rannacher's two Euler
 //      half steps for step 0). Each stage reverse is three cross-lane affine
 //      scans (back-substitution carry, rhs adjoint, denominator adjoint);
 //   4. the per-node adjoints of the Jacobian bands are contracted with the
@@ -21,10 +24,14 @@
 //      b = r - q - sigma^2/2, c = -r) and summed over the group:
 //      dV/dsigma = sigma (dV/da - dV/db), dV/dr = dV/db - dV/dc + boundary,
 //      dV/dq = -dV/db; dV/dK = (V - S Delta)/K (strike-free PDE).
-// Cost: one forward, one recompute and one reverse march (about three stage
-// solves' worth of scans per step).
+// Cost: one forward and one reverse march (plus one recompute march for
+// checkpointed jobs). Both modes give the same forward bits (a stage
+// factorization is a function of the stage's lock pattern and weight only);
+// the sensitivities differ by rounding only, as checkpointed jobs also settle
+// the accumulated denominator-adjoint sums at each segment start.
 #include "american_device.hpp"
 
+#include <cstdio>
 #include <cstdlib>
 
 namespace mango_amd {
@@ -103,7 +110,8 @@ __device__ __forceinline__ void adjoint_one(const PdeJob* __restrict__ jobs, int
     const int64_t N = P.steps(job);
     const double dt = job.maturity / static_cast<double>(N);
     P.set_dt(dt);
-    int64_t S = static_cast<int64_t>(ceil(sqrt(static_cast<double>(N))));
+    const bool whole = N <= seg_cap;  // the forward writes the whole tape
+    int64_t S = whole ? N : static_cast<int64_t>(ceil(sqrt(static_cast<double>(N))));
     if (S < 1) S = 1;
     const int64_t n_ck = (N + S - 1) / S;
     bool ok = P.valid && N > 0 && S <= seg_cap && n_ck <= ck_cap;
@@ -119,10 +127,12 @@ __device__ __forceinline__ void adjoint_one(const PdeJob* __restrict__ jobs, int
 #pragma unroll
     for (int k = 0; k < M; ++k) up[k] = mid[k] = u[k];
     for (int64_t s = 0; ok && s < N; ++s) {
-        if (s % S == 0) row_store<K, M>(ck + (s / S) * NMAX, lane, u);
+        if (whole) row_store<K, M>(tr + (2 * s) * NMAX, lane, u);
+        else if (s % S == 0) row_store<K, M>(ck + (s / S) * NMAX, lane, u);
 #pragma unroll
         for (int k = 0; k < M; ++k) up[k] = u[k];
         ok = P.advance(s, dt, u, mid, c0, c1);
+        if (whole) row_store<K, M>(tr + (2 * s + 1) * NMAX, lane, mid);
     }
     if (N == 1) {
 #pragma unroll
@@ -168,15 +178,17 @@ __device__ __forceinline__ void adjoint_one(const PdeJob* __restrict__ jobs, int
     for (int64_t seg = n_ck - 1; ok && seg >= 0; --seg) {
         const int64_t s0 = seg * S;
         const int64_t s1 = (s0 + S < N) ? s0 + S : N;
-        // the recompute may refactorize: settle the accumulated sources first
-        P.flush(c0, a0, gl, gd, gu);
-        P.flush(c1, a1, gl, gd, gu);
-        double w[M];
-        row_load<K, M>(ck + seg * NMAX, lane, w);
-        for (int64_t s = s0; ok && s < s1; ++s) {
-            row_store<K, M>(tr + (2 * (s - s0)) * NMAX, lane, w);
-            ok = P.advance(s, dt, w, mid, c0, c1);
-            row_store<K, M>(tr + (2 * (s - s0) + 1) * NMAX, lane, mid);
+        if (!whole) {
+            // the recompute may refactorize: settle the accumulated sources first
+            P.flush(c0, a0, gl, gd, gu);
+            P.flush(c1, a1, gl, gd, gu);
+            double w[M];
+            row_load<K, M>(ck + seg * NMAX, lane, w);
+            for (int64_t s = s0; ok && s < s1; ++s) {
+                row_store<K, M>(tr + (2 * (s - s0)) * NMAX, lane, w);
+                ok = P.advance(s, dt, w, mid, c0, c1);
+                row_store<K, M>(tr + (2 * (s - s0) + 1) * NMAX, lane, mid);
+            }
         }
         for (int64_t s = s1 - 1; ok && s >= s0; --s) {
             double un[M], um[M], rhs[M], rbf[M], unb[M];
@@ -298,24 +310,39 @@ int resident_blocks() {
 
 }  // namespace
 
-#define MANGO_ADJOINT_VARIANTS(X) X(32, 4) X(64, 2) X(64, 4) X(64, 7)
+#define MANGO_ADJOINT_VARIANTS(X) X(16, 7) X(32, 4) X(64, 2) X(64, 4) X(64, 7)
 
-// $MANGO_AMD_ADJOINT_LANES=64 selects one option per wavefront up to 128
-// points (64 x 2) instead of two (32 x 4).
+// $MANGO_AMD_ADJOINT_VARIANT="lanes,nodes" forces a compiled variant where
+// the grid fits it (timing and tuning).
 SolveVariant pick_adjoint_variant(int64_t n_space) {
-    static const bool wide = [] {
-        const char* v = std::getenv("MANGO_AMD_ADJOINT_LANES");
-        return v && std::atoi(v) == 64;
+    static const SolveVariant forced = [] {
+        SolveVariant f{0, 0};
+        const char* v = std::getenv("MANGO_AMD_ADJOINT_VARIANT");
+        if (v) std::sscanf(v, "%d,%d", &f.lanes, &f.nodes);
+        return f;
     }();
-    if (n_space <= 128) return wide ? SolveVariant{64, 2} : SolveVariant{32, 4};
+    if (forced.lanes > 0 && static_cast<int64_t>(forced.lanes) * forced.nodes >= n_space) {
+#define MANGO_FORCED(K, M) \
+        if (forced.lanes == K && forced.nodes == M) return forced;
+        MANGO_ADJOINT_VARIANTS(MANGO_FORCED)
+#undef MANGO_FORCED
+    }
+    if (n_space <= 112) return {16, 7};
+    if (n_space <= 128) return {32, 4};
     if (n_space <= 256) return {64, 4};
     return {64, 7};
 }
 
 int max_adjoint_nodes() { return 64 * 7; }
 
-void adjoint_rows(int64_t max_steps, int* seg_cap, int* ck_cap) {
-    const int64_t s = static_cast<int64_t>(std::ceil(std::sqrt(static_cast<double>(std::max<int64_t>(max_steps, 1)))));
+void adjoint_rows(int64_t max_steps, bool whole_tape, int* seg_cap, int* ck_cap) {
+    max_steps = std::max<int64_t>(max_steps, 1);
+    if (whole_tape) {
+        *seg_cap = static_cast<int>(max_steps);
+        *ck_cap = 1;
+        return;
+    }
+    const int64_t s = static_cast<int64_t>(std::ceil(std::sqrt(static_cast<double>(max_steps))));
     *seg_cap = static_cast<int>(s);
     *ck_cap = static_cast<int>(s + 1);  // ceil(N / ceil(sqrt(N))) <= ceil(sqrt(N)) + 1
 }

@@ -149,8 +149,10 @@ struct AdjointGreeks {  // layout of MangoAdjointGreeks
 // Variant serving n_space in the adjoint kernel (a subset of the solve variants).
 SolveVariant pick_adjoint_variant(int64_t n_space);
 int max_adjoint_nodes();
-// Checkpoint and segment rows one group needs for at most max_steps steps.
-void adjoint_rows(int64_t max_steps, int* seg_cap, int* ck_cap);
+// Checkpoint and segment rows one group needs for at most max_steps steps:
+// the whole tape (the forward stores every step, no recompute) or sqrt(N)
+// checkpoints and one segment's tape (recompute per segment).
+void adjoint_rows(int64_t max_steps, bool whole_tape, int* seg_cap, int* ck_cap);
 // Persistent groups launched for n_jobs (each owns one scratch slot of
 // (ck_cap + 2 seg_cap) * lanes * nodes doubles).
 int adjoint_groups(const SolveVariant& v, int n_jobs);

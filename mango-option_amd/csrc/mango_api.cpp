@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <fstream>
 #include <sstream>
@@ -972,6 +973,8 @@ struct AdjointScratch {
     }
 };
 AdjointScratch g_adj;
+constexpr uint64_t kAdjointTapeDefault = 8ull << 30;
+std::atomic<uint64_t> g_adj_budget{kAdjointTapeDefault};
 
 MangoStatus check_device_grid(const MangoGridConfig* grid, MangoError* err) {
     if (grid->mode == MANGO_GRID_MODE_EXPLICIT && (grid->spec.n_points < 3 || grid->n_time == 0)) {
@@ -1016,15 +1019,24 @@ MangoStatus device_adjoint(const MangoDeviceOption* d_opts, uint64_t n, const Ma
                                   ? std::max<int64_t>(static_cast<int64_t>(grid->accuracy.max_time_steps),
                                                       static_cast<int64_t>(grid->min_time_steps))
                                   : static_cast<int64_t>(grid->n_time);
+    // whole-step tapes when they fit the budget (no recompute), else sqrt(N)
+    // checkpoints; the scratch is shared by the bins, which run in order
     int seg_cap = 0, ck_cap = 0;
-    adjoint_rows(max_steps, &seg_cap, &ck_cap);
-    size_t need = 0;
-    for (int b = 0; b < bins.n_bins; ++b) {
-        const size_t groups = static_cast<size_t>(adjoint_groups(variants[b], ni));
-        need = std::max(need, groups * static_cast<size_t>(ck_cap + 2 * seg_cap) *
-                                  static_cast<size_t>(variants[b].lanes * variants[b].nodes));
+    const auto rows_need = [&](bool whole) {
+        adjoint_rows(max_steps, whole, &seg_cap, &ck_cap);
+        size_t need = 0;
+        for (int b = 0; b < bins.n_bins; ++b) {
+            const size_t groups = static_cast<size_t>(adjoint_groups(variants[b], ni));
+            need = std::max(need, groups * static_cast<size_t>(ck_cap + 2 * static_cast<size_t>(seg_cap)) *
+                                      static_cast<size_t>(variants[b].lanes * variants[b].nodes));
+        }
+        return need;
+    };
+    size_t need = rows_need(true);
+    if (need * sizeof(double) > g_adj_budget.load() || !g_adj.ensure(need)) {
+        need = rows_need(false);
+        if (!g_adj.ensure(need)) { set_err(err, MANGO_ERR_SOLVER, "hipMalloc failed (adjoint scratch)"); return MANGO_ERR_SOLVER; }
     }
-    if (!g_adj.ensure(need)) { set_err(err, MANGO_ERR_SOLVER, "hipMalloc failed (adjoint scratch)"); return MANGO_ERR_SOLVER; }
     bool ok = launch_fill_i32(g_queue.ints, QueueScratch::kInts, 0, stream) == 0;
     ok = ok && launch_plan_binned(reinterpret_cast<const DeviceOptionRecord*>(d_opts), ni, plan, bins, g_queue.jobs,
                                   g_queue.keys, g_queue.hist(), g_queue.cursor(), g_queue.bin_start(),
@@ -1581,6 +1593,10 @@ MangoStatus mango_plan_grid_sizes(const MangoDeviceOption* opts, uint64_t n, con
             n_time[i] = static_cast<uint64_t>(std::max<int64_t>(nt, static_cast<int64_t>(grid->min_time_steps)));
     }
     return MANGO_OK;
+}
+
+uint64_t mango_adjoint_tape_budget(uint64_t bytes) {
+    return g_adj_budget.exchange(bytes ? bytes : kAdjointTapeDefault);
 }
 
 void mango_profile_enable(int32_t enable) {

@@ -281,6 +281,7 @@ SIGNATURES = {
                                                        _P(MangoError)]),
     "mango_price_american_adjoint_device": (C.c_int32, [C.c_void_p, C.c_uint64, _P(MangoGridConfig), C.c_void_p,
                                                         C.c_void_p, C.c_void_p, _P(MangoError)]),
+    "mango_adjoint_tape_budget": (C.c_uint64, [C.c_uint64]),
     "mango_plan_grid_sizes": (C.c_int32, [C.c_void_p, C.c_uint64, _P(MangoGridConfig), _P(C.c_uint64),
                                           _P(C.c_uint64), _P(MangoError)]),
     "mango_tuning_set": (C.c_int32, [_P(MangoVariantRule), C.c_uint64, _P(MangoError)]),

@@ -125,3 +125,28 @@ def test_adjoint_host_entry_and_invalid_slots():
     ok = s == 0
     assert np.array_equal(ds[ok], s[ok])
     assert np.array_equal(d[ok], h[ok])
+
+
+def test_whole_tape_and_checkpointed_modes_agree():
+    # the budget decides between whole-step tapes (no recompute) and sqrt(N)
+    # checkpoints. A stage factorization depends only on its lock pattern and
+    # weight, so the forward values and Greeks are the same bits; the
+    # sensitivities differ only by the rounding of where the accumulated
+    # denominator-adjoint sums are settled (also at segment starts when
+    # checkpointed)
+    _say("test_whole_tape_and_checkpointed_modes_agree: 64 fixed-grid puts + 48 uneven options")
+    lib = _abi.load_library()
+    cases = [(W.c1_puts(64, seed=31), _fixed_cfg()), (W.c3_uneven(48, seed=32), _uneven_cfg())]
+    try:
+        for k, (opts, cfg) in enumerate(cases):
+            lib.mango_adjoint_tape_budget(0)
+            whole, st_w = _device_adjoint(opts, cfg)
+            lib.mango_adjoint_tape_budget(1)  # below any tape: checkpoints
+            ck, st_c = _device_adjoint(opts, cfg)
+            assert np.array_equal(st_w, st_c), k
+            assert (st_w == 0).all(), k
+            assert np.array_equal(whole[:, :4].view(np.uint64), ck[:, :4].view(np.uint64)), k
+            np.testing.assert_allclose(whole[:, 4:], ck[:, 4:], rtol=1e-9, atol=1e-11)
+            _say(f"  case {k}: max |difference| of the sensitivities {np.abs(whole - ck).max():.2e}")
+    finally:
+        lib.mango_adjoint_tape_budget(0)
