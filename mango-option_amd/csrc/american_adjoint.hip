@@ -122,21 +122,35 @@ __device__ __forceinline__ void adjoint_one(const PdeJob* __restrict__ jobs, int
     StageCache<K, M> c0, c1;
     c0.invalidate();
     c1.invalidate();
+    // The Rannacher start is peeled off the step loop, whose body is then one
+    // TR-BDF2 step (a smaller live set than a loop holding both kinds).
     double u[M], mid[M], up[M];
     P.initial(u);
 #pragma unroll
     for (int k = 0; k < M; ++k) up[k] = mid[k] = u[k];
-    for (int64_t s = 0; ok && s < N; ++s) {
-        if (whole) row_store<K, M>(tr + (2 * s) * NMAX, lane, u);
-        else if (s % S == 0) row_store<K, M>(ck + (s / S) * NMAX, lane, u);
+    const TrConstants trc;
+    const double w1 = trc.w1(dt), w2 = trc.w2(dt);
+    if (ok) {
+        row_store<K, M>(whole ? tr : ck, lane, u);
+        ok = P.rannacher(dt, u, mid, c0);
+        if (whole) row_store<K, M>(tr + NMAX, lane, mid);
 #pragma unroll
-        for (int k = 0; k < M; ++k) up[k] = u[k];
-        ok = P.advance(s, dt, u, mid, c0, c1);
-        if (whole) row_store<K, M>(tr + (2 * s + 1) * NMAX, lane, mid);
+        for (int k = 0; k < M; ++k) up[k] = mid[k];  // N == 1: the half step is u_prev
     }
-    if (N == 1) {
+    int64_t next_ck = S;
+    for (int64_t s = 1; ok && s < N; ++s) {
+        if (whole) {
+            row_store<K, M>(tr + (2 * s) * NMAX, lane, u);
+        } else if (s == next_ck) {
+            row_store<K, M>(ck + (s / S) * NMAX, lane, u);
+            next_ck += S;
+        }
+        if (s == N - 1) {
 #pragma unroll
-        for (int k = 0; k < M; ++k) up[k] = mid[k];  // the Rannacher start leaves its half step as u_prev
+            for (int k = 0; k < M; ++k) up[k] = u[k];
+        }
+        ok = P.trbdf2(s, dt, w1, w2, u, mid, c0, c1);
+        if (whole) row_store<K, M>(tr + (2 * s + 1) * NMAX, lane, mid);
     }
 
     // ---- Greeks at spot (as the solve kernel) and the seed dV/du^N ----
@@ -167,7 +181,6 @@ __device__ __forceinline__ void adjoint_one(const PdeJob* __restrict__ jobs, int
     }
 
     // ---- 3. segments in reverse: recompute, then reverse the steps ----
-    const TrConstants trc;
     double gl[M], gd[M], gu[M], unext[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) { gl[k] = gd[k] = gu[k] = 0.0; unext[k] = u[k]; }
@@ -186,44 +199,48 @@ __device__ __forceinline__ void adjoint_one(const PdeJob* __restrict__ jobs, int
             row_load<K, M>(ck + seg * NMAX, lane, w);
             for (int64_t s = s0; ok && s < s1; ++s) {
                 row_store<K, M>(tr + (2 * (s - s0)) * NMAX, lane, w);
-                ok = P.advance(s, dt, w, mid, c0, c1);
+                ok = s == 0 ? P.rannacher(dt, w, mid, c0) : P.trbdf2(s, dt, w1, w2, w, mid, c0, c1);
                 row_store<K, M>(tr + (2 * (s - s0) + 1) * NMAX, lane, mid);
             }
         }
-        for (int64_t s = s1 - 1; ok && s >= s0; --s) {
-            double un[M], um[M], rhs[M], rbf[M], unb[M];
+        // TR-BDF2 steps of the segment, last to first (step 0 after the loop)
+        const int64_t s_lo = s0 > 0 ? s0 : 1;
+        for (int64_t s = s1 - 1; ok && s >= s_lo; --s) {
+            double un[M], um[M], rhs[M], rbf[M], unb[M], umb[M];
             row_load<K, M>(tr + (2 * (s - s0)) * NMAX, lane, un);
             row_load<K, M>(tr + (2 * (s - s0) + 1) * NMAX, lane, um);
-            if (s == 0) {
-                const double half_dt = 0.5 * dt;
-                P.flush(c0, a0, gl, gd, gu);
-                c0.invalidate();
-                const double d1 = exp(-P.r * (0.0 + dt)), dh = exp(-P.r * (0.0 + half_dt));
-                ok = P.stage_reverse(0.0 + dt, d1, half_dt, um, um, c0, a0, unext, ub, rbf, gl, gd, gu, rbc);
-                ok = ok && P.stage_reverse(0.0 + half_dt, dh, half_dt, un, un, c0, a0, um, rbf, unb, gl, gd, gu,
-                                           rbc);
-                P.flush(c0, a0, gl, gd, gu);
-                c0.invalidate();
-            } else {
-                const double t = static_cast<double>(s) * dt;
-                const double t_next = static_cast<double>(s + 1) * dt;
-                const double disc = P.put ? 0.0 : exp(-P.r * t);  // as the forward's advance()
-                const double w1 = trc.w1(dt);
+            const double t = static_cast<double>(s) * dt;
+            const double t_next = static_cast<double>(s + 1) * dt;
+            double disc = 0.0;
+            if (!P.put) disc = exp(-P.r * t);  // as the forward's trbdf2()
 #pragma unroll
-                for (int k = 0; k < M; ++k) rhs[k] = fma(trc.alpha, um[k], trc.beta * un[k]);
-                ok = P.stage_reverse(t_next, disc * P.e_dt, trc.w2(dt), rhs, um, c1, a1, unext, ub, rbf, gl, gd, gu,
-                                     rbc);
-                double umb[M];
+            for (int k = 0; k < M; ++k) rhs[k] = fma(trc.alpha, um[k], trc.beta * un[k]);
+            ok = P.stage_reverse(t_next, disc * P.e_dt, w2, rhs, um, c1, a1, unext, ub, rbf, gl, gd, gu, rbc);
 #pragma unroll
-                for (int k = 0; k < M; ++k) {
-                    umb[k] = trc.alpha * rbf[k];
-                    unb[k] = trc.beta * rbf[k];
-                }
-                P.stage1_rhs(w1, un, rhs);
-                ok = ok && P.stage_reverse(t + trc.gamma * dt, disc * P.e_g, w1, rhs, un, c0, a0, um, umb, rbf, gl,
-                                           gd, gu, rbc);
-                P.stage1_rhs_reverse(w1, un, rbf, unb, gl, gd, gu);
+            for (int k = 0; k < M; ++k) {
+                umb[k] = trc.alpha * rbf[k];
+                unb[k] = trc.beta * rbf[k];
             }
+            P.stage1_rhs(w1, un, rhs);
+            ok = ok && P.stage_reverse(t + trc.gamma * dt, disc * P.e_g, w1, rhs, un, c0, a0, um, umb, rbf, gl, gd,
+                                       gu, rbc);
+            P.stage1_rhs_reverse(w1, un, rbf, unb, gl, gd, gu);
+#pragma unroll
+            for (int k = 0; k < M; ++k) { ub[k] = unb[k]; unext[k] = un[k]; }
+        }
+        if (ok && s0 == 0) {
+            // the Rannacher start reversed: its two Euler half steps
+            double un[M], um[M], rbf[M], unb[M];
+            row_load<K, M>(tr, lane, un);
+            row_load<K, M>(tr + NMAX, lane, um);
+            const double half_dt = 0.5 * dt;
+            P.flush(c0, a0, gl, gd, gu);
+            c0.invalidate();
+            const double d1 = exp(-P.r * (0.0 + dt)), dh = exp(-P.r * (0.0 + half_dt));
+            ok = P.stage_reverse(0.0 + dt, d1, half_dt, um, um, c0, a0, unext, ub, rbf, gl, gd, gu, rbc);
+            ok = ok && P.stage_reverse(0.0 + half_dt, dh, half_dt, un, un, c0, a0, um, rbf, unb, gl, gd, gu, rbc);
+            P.flush(c0, a0, gl, gd, gu);
+            c0.invalidate();
 #pragma unroll
             for (int k = 0; k < M; ++k) { ub[k] = unb[k]; unext[k] = un[k]; }
         }

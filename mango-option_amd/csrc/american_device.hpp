@@ -924,29 +924,36 @@ struct OptionPde {
         }
     }
 
-    // Time step `step` of n: step 0 is the Rannacher start (two implicit-Euler
-    // half steps, pde_solver.hpp:413-447), later steps TR-BDF2. u is advanced
-    // in place; mid receives the intermediate state (half step / u*).
-    // c0 serves the Euler half steps (then is invalidated) and stage 1, c1 stage 2.
-    __device__ __forceinline__ bool advance(int64_t step, double dt, double (&u)[M], double (&mid)[M],
-                                            StageCache<K, M>& c0, StageCache<K, M>& c1) const {
+    // Step 0: the Rannacher start, two implicit-Euler half steps
+    // (pde_solver.hpp:413-447). u is advanced in place; mid receives the half
+    // step. c0 serves both half steps and is invalidated before and after
+    // (the Euler weight differs from stage 1's).
+    __device__ __forceinline__ bool rannacher(double dt, double (&u)[M], double (&mid)[M],
+                                              StageCache<K, M>& c0) const {
+        const double half_dt = 0.5 * dt;
+        double un[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) un[k] = u[k];
+        c0.invalidate();
+        bool ok = stage(g_right(0.0 + half_dt), half_dt, un, un, c0, u);
+#pragma unroll
+        for (int k = 0; k < M; ++k) mid[k] = u[k];
+        ok = ok && stage(g_right(0.0 + dt), half_dt, mid, mid, c0, u);
+        c0.invalidate();
+        return ok;
+    }
+
+    // Step `step` >= 1: TR-BDF2 (pde_solver.hpp:342-395) with the stage
+    // weights w1 = gamma dt / 2 and w2 = (1 - gamma) dt / (2 - gamma). u is
+    // advanced in place; mid receives u*. c0 serves stage 1, c1 stage 2.
+    __device__ __forceinline__ bool trbdf2(int64_t step, double dt, double w1, double w2, double (&u)[M],
+                                           double (&mid)[M], StageCache<K, M>& c0, StageCache<K, M>& c1) const {
         const TrConstants tr;
         double un[M], rhs[M];
 #pragma unroll
         for (int k = 0; k < M; ++k) un[k] = u[k];
-        if (step == 0) {
-            const double half_dt = 0.5 * dt;
-            c0.invalidate();
-            bool ok = stage(g_right(0.0 + half_dt), half_dt, un, un, c0, u);
-#pragma unroll
-            for (int k = 0; k < M; ++k) mid[k] = u[k];
-            ok = ok && stage(g_right(0.0 + dt), half_dt, mid, mid, c0, u);
-            c0.invalidate();
-            return ok;
-        }
-        const double t = static_cast<double>(step) * dt;
-        const double disc = put ? 0.0 : exp(-r * t);
-        const double w1 = tr.w1(dt);
+        double disc = 0.0;
+        if (!put) disc = exp(-r * (static_cast<double>(step) * dt));
         stage1_rhs(w1, un, rhs);
         if (!stage(g_right_d(disc * e_g), w1, rhs, un, c0, u)) return false;
 #pragma unroll
@@ -954,7 +961,15 @@ struct OptionPde {
             mid[k] = u[k];
             rhs[k] = fma(tr.alpha, u[k], tr.beta * un[k]);
         }
-        return stage(g_right_d(disc * e_dt), tr.w2(dt), rhs, mid, c1, u);
+        return stage(g_right_d(disc * e_dt), w2, rhs, mid, c1, u);
+    }
+
+    // Time step `step` of n: the Rannacher start at step 0, TR-BDF2 after.
+    __device__ __forceinline__ bool advance(int64_t step, double dt, double (&u)[M], double (&mid)[M],
+                                            StageCache<K, M>& c0, StageCache<K, M>& c1) const {
+        if (step == 0) return rannacher(dt, u, mid, c0);
+        const TrConstants tr;
+        return trbdf2(step, dt, tr.w1(dt), tr.w2(dt), u, mid, c0, c1);
     }
 
     // Lock pattern eliminate() will use for this guess.
