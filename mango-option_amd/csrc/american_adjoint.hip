@@ -203,12 +203,21 @@ __device__ __forceinline__ void adjoint_one(const PdeJob* __restrict__ jobs, int
                 row_store<K, M>(tr + (2 * (s - s0) + 1) * NMAX, lane, mid);
             }
         }
-        // TR-BDF2 steps of the segment, last to first (step 0 after the loop)
+        // TR-BDF2 steps of the segment, last to first (step 0 after the
+        // loop). The tape rows of step s - 1 are loaded while step s is
+        // reversed, so the HBM latency is off the critical path.
         const int64_t s_lo = s0 > 0 ? s0 : 1;
+        double pn[M], pm[M];
+        row_load<K, M>(tr + (2 * (s1 - 1 - s0)) * NMAX, lane, pn);
+        row_load<K, M>(tr + (2 * (s1 - 1 - s0) + 1) * NMAX, lane, pm);
         for (int64_t s = s1 - 1; ok && s >= s_lo; --s) {
             double un[M], um[M], rhs[M], rbf[M], unb[M], umb[M];
-            row_load<K, M>(tr + (2 * (s - s0)) * NMAX, lane, un);
-            row_load<K, M>(tr + (2 * (s - s0) + 1) * NMAX, lane, um);
+#pragma unroll
+            for (int k = 0; k < M; ++k) { un[k] = pn[k]; um[k] = pm[k]; }
+            if (s > s0) {
+                row_load<K, M>(tr + (2 * (s - 1 - s0)) * NMAX, lane, pn);
+                row_load<K, M>(tr + (2 * (s - 1 - s0) + 1) * NMAX, lane, pm);
+            }
             const double t = static_cast<double>(s) * dt;
             const double t_next = static_cast<double>(s + 1) * dt;
             double disc = 0.0;

@@ -89,8 +89,15 @@ __global__ __launch_bounds__(kSolveBlock) void table_snapshot_kernel(const PdeJo
         }
     };
     if (ok) record(0);
-    for (int64_t s = 0; ok && s < N; ++s) {
-        ok = P.advance(s, dt, u, mid, c0, c1);
+    if (ok) {
+        ok = P.rannacher(dt, u, mid, c0);
+        if (ok) record(1);
+    }
+    // one TR-BDF2 step per loop body (the Rannacher start peeled off above)
+    const TrConstants trc;
+    const double w1 = trc.w1(dt), w2 = trc.w2(dt);
+    for (int64_t s = 1; ok && s < N; ++s) {
+        ok = P.trbdf2(s, dt, w1, w2, u, mid, c0, c1);
         if (ok) record(s + 1);
     }
     if (lane == 0) status_out[j] = !P.valid ? kStatusValidation : (ok && next == ns ? kStatusOk : kStatusSolver);
