@@ -395,7 +395,7 @@ def run_fdm_iv(g: Gpu, args):
             "queries": n, "queries_per_s": n * args.fdm_iv_steps / el, "ms_per_step": 1e3 * el / args.fdm_iv_steps,
             "converged": int(ok.sum()), "mean_brent_iterations": float(its[ok].mean()) if ok.any() else None,
             "max_abs_iv_err_vs_true_sigma": float(np.max(np.abs(iv[ok] - batch.volatility[ok]))) if ok.any() else None,
-            "_qs": qs, "_iv": iv, "_stat": stat}
+            "_qs": qs, "_iv": iv, "_stat": stat, "_iters": its}
 
 
 def args_fdm_check(fdm) -> int:
@@ -488,11 +488,24 @@ def cpu_baseline(chains, batched, seconds: float, adjoint=None, table=None, even
         fdm_s = time.perf_counter() - t0
         rst = np.array([r[0] for r in ref])
         riv = np.array([r[1] for r in ref])
+        rit = np.array([r[3] for r in ref])
         ok = (rst == 0) & (fdm["_stat"][:k] == 0)
+        div = np.abs(riv - fdm["_iv"][:k])
+        # the IV difference carried back to price: |dsigma| x the Black-Scholes
+        # vega at the oracle's sigma (the price bar of 1e-9 applies to it)
+        vega = np.zeros(k)
+        for i in np.nonzero(ok)[0]:
+            q, sg = qs[i], riv[i]
+            d1 = (np.log(q.spot / q.strike) + (q.rate_const - q.dividend_yield + 0.5 * sg * sg) * q.maturity) / (
+                sg * np.sqrt(q.maturity))
+            vega[i] = q.spot * np.exp(-q.dividend_yield * q.maturity) * np.exp(-0.5 * d1 * d1) / np.sqrt(
+                2.0 * np.pi) * np.sqrt(q.maturity)
         out["fdm_iv"] = {"value": k / fdm_s, "unit": "queries/s", "cores": threads,
                          "sample": f"the first {k} FDM IV queries, oracle IVSolver, {threads} threads",
                          "parity_status_mismatches": int(np.sum(rst != fdm["_stat"][:k])),
-                         "parity_max_abs_iv_err": float(np.max(np.abs(riv[ok] - fdm["_iv"][:k][ok]), initial=0.0))}
+                         "parity_iteration_mismatches": int(np.sum(ok & (rit != fdm["_iters"][:k]))),
+                         "parity_max_abs_iv_err": float(np.max(div[ok], initial=0.0)),
+                         "parity_max_iv_err_times_vega": float(np.max((div * vega)[ok], initial=0.0))}
     if adjoint is not None:
         ref, _ = O.adjoint_batch(adjoint["_batch"], adjoint["_cfg"], n_threads=threads)
         sens = adjoint["_out"][:, 4:8]
