@@ -84,15 +84,17 @@ __device__ void value_seed(const double* x, const double* y, int n, double spot,
     }
 }
 
+// Tape rows are written once and read back once, long after: streaming
+// (non-temporal) accesses keep them from displacing cached data.
 template <int K, int M>
 __device__ __forceinline__ void row_store(double* __restrict__ row, int lane, const double (&v)[M]) {
 #pragma unroll
-    for (int k = 0; k < M; ++k) row[k * K + lane] = v[k];
+    for (int k = 0; k < M; ++k) __builtin_nontemporal_store(v[k], row + k * K + lane);
 }
 template <int K, int M>
 __device__ __forceinline__ void row_load(const double* __restrict__ row, int lane, double (&v)[M]) {
 #pragma unroll
-    for (int k = 0; k < M; ++k) v[k] = row[k * K + lane];
+    for (int k = 0; k < M; ++k) v[k] = __builtin_nontemporal_load(row + k * K + lane);
 }
 
 template <int K, int M>
