@@ -721,7 +721,8 @@ MangoStatus device_chains(const MangoDeviceOption* d_opts, uint64_t n, const int
     StateOut so{g_chain.st, g_chain.st + plane, g_chain.st + 2 * plane, g_chain.st + 3 * plane,
                 g_chain.st + 4 * plane, g_chain.ns, g_chain.dt, g_chain.stride};
     const int ni = static_cast<int>(n);
-    bool ok = hipMemsetAsync(g_chain.job_count, 0, sizeof(int32_t), stream) == hipSuccess;
+    // a fill kernel, not hipMemsetAsync: callers may capture this call in a graph
+    bool ok = launch_fill_i32(g_chain.job_count, 1, 0, stream) == 0;
     ok = ok && launch_plan_chains(reinterpret_cast<const DeviceOptionRecord*>(d_opts), d_seg_offsets,
                                   static_cast<int>(n_segments), plan, g_chain.jobs, g_chain.job_count, g_chain.copts,
                                   g_chain.rep, stream) == 0;

@@ -155,3 +155,46 @@ def test_device_chains_match_oracle():
             sel = np.where(sub.option_type == typ)[0]
             gr, sr, _, _ = O.price_batch(sub.take(sel), True, ecfg, n_threads=8)
             _check(g[idx[sel]], s[idx[sel]], gr, sr, f"chain{c}/{typ}")
+
+
+def test_device_chains_replay_from_a_captured_graph():
+    # a caller may capture the chains entry point in a graph: it clears its job
+    # counter with a kernel (no memset node), so every replay, with eager work
+    # in between, reproduces the eager call bit for bit
+    _say("test_device_chains_replay_from_a_captured_graph: 8 chains x 64 strikes, 3 replays")
+    torch = pytest.importorskip("torch")
+    opts, _ = W.c2_chains(8, 64, seed=3)
+    seg = np.arange(0, len(opts) + 1, 64, dtype=np.int64)
+    dev = Dev(opts)
+    d_seg = torch.from_numpy(seg).cuda()
+    acc = M.GridAccuracyParams(min_spatial_points=101, max_spatial_points=101)
+    cfg = M.grid_config(_abi.MANGO_GRID_MODE_ACCURACY_FIXED_TIME, acc, n_time=498)
+    lib = _abi.load_library()
+    err = _abi.MangoError()
+
+    def call(stream):
+        rc = lib.mango_price_american_chains_device(dev.d_opts.data_ptr(), dev.n, d_seg.data_ptr(), len(seg) - 1,
+                                                    C.byref(cfg), dev.d_out.data_ptr(), dev.d_st.data_ptr(), stream,
+                                                    C.byref(err))
+        assert rc == 0, err.message.decode()
+
+    call(dev.stream)  # eager; also sizes the library's scratch
+    g0, s0 = dev.result()
+    assert (s0 == 0).all()
+    side = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(side):
+        graph.capture_begin()
+        call(C.c_void_p(side.cuda_stream))
+        graph.capture_end()
+    torch.cuda.synchronize()
+    for r in range(3):
+        dev.d_out.fill_(float("nan"))
+        dev.d_st.fill_(-1)
+        torch.cuda.synchronize()
+        graph.replay()
+        g, s = dev.result()
+        assert np.array_equal(s, s0), r
+        assert np.array_equal(g.view(np.uint64), g0.view(np.uint64)), (r, np.nanmax(np.abs(g - g0)))
+        _say(f"  replay {r}: identical")
