@@ -133,7 +133,7 @@ struct CurveCursor {
 };
 
 template <int K, int M>
-__device__ void solve_event_job(const PdeJob& job, const double* __restrict__ pool, const int lane,
+__device__ __forceinline__ void solve_event_job(const PdeJob& job, const double* __restrict__ pool, const int lane,
                                 double* __restrict__ xs, double* __restrict__ ys, double* __restrict__ cps,
                                 double* __restrict__ cs, Greeks* __restrict__ greeks_out,
                                 int32_t* __restrict__ status_out, const StateOut& state) {
