@@ -22,6 +22,9 @@
 #include "american_device.hpp"
 #include "mango_events.hpp"
 
+#include <cstdio>
+#include <cstdlib>
+
 namespace mango_amd {
 namespace {
 
@@ -386,7 +389,20 @@ __global__ __launch_bounds__(kSolveBlock) void american_events_kernel(const PdeJ
 
 // (lanes, nodes per lane) of an event launch: several jobs per wavefront for
 // launches that fill the chip, one job per wavefront for small ones.
+// $MANGO_AMD_EVENTS_VARIANT="lanes,nodes" forces a compiled variant where
+// the grid fits it (timing and tuning).
 SolveVariant events_variant_for(int64_t n_points, int64_t n_jobs) {
+    static const SolveVariant forced = [] {
+        SolveVariant f{0, 0};
+        const char* v = std::getenv("MANGO_AMD_EVENTS_VARIANT");
+        if (v) std::sscanf(v, "%d,%d", &f.lanes, &f.nodes);
+        return f;
+    }();
+    if (forced.lanes > 0 && static_cast<int64_t>(forced.lanes) * forced.nodes >= n_points) {
+        if ((forced.lanes == 16 && forced.nodes == 7) || (forced.lanes == 32 && forced.nodes == 4) ||
+            (forced.lanes == 64 && (forced.nodes == 2 || forced.nodes == 4 || forced.nodes == 7)))
+            return forced;
+    }
     if (n_jobs > kLatencyJobs) {
         if (n_points <= 16 * 7) return {16, 7};
         if (n_points <= 32 * 4) return {32, 4};
