@@ -530,10 +530,13 @@ __global__ __launch_bounds__(64) void plan_chains_kernel(const DeviceOptionRecor
     for (int64_t i = b + lane; i < e; i += 64) {
         const DeviceOptionRecord o = opts[i];
         if (!valid(o)) continue;
-        const GridDesc g = estimate_spatial(o.spot, o.strike, o.maturity, o.volatility, plan.accuracy);
-        gx_min = smin(gx_min, g.x_min);
-        gx_max = smax(gx_max, g.x_max);
-        gnx = gnx > g.n_points ? gnx : g.n_points;
+        // only the domain and point count of each option's grid enter the union
+        double x0, lo, hi;
+        int64_t nx;
+        spatial_bounds(o.spot, o.strike, o.maturity, o.volatility, plan.accuracy, 0.0, x0, lo, hi, nx);
+        gx_min = smin(gx_min, lo);
+        gx_max = smax(gx_max, hi);
+        gnx = gnx > nx ? gnx : nx;
     }
     gx_min = group_min<64>(gx_min);
     gx_max = group_max<64>(gx_max);
@@ -562,8 +565,9 @@ __global__ __launch_bounds__(64) void plan_chains_kernel(const DeviceOptionRecor
         }
         return shared;
     };
+    // fields written straight to the record (no private staging copy)
     auto make_job = [&](const DeviceOptionRecord& o, const GridDesc& g, int32_t slot) {
-        PdeJob jb;
+        PdeJob& jb = jobs[slot];
         jb.maturity = o.maturity; jb.volatility = o.volatility; jb.rate = o.rate;
         jb.dividend_yield = o.dividend_yield;
         jb.spot = 1.0; jb.strike = 1.0; jb.c_t = plan.accuracy.c_t; jb.ext = 0;
@@ -574,7 +578,6 @@ __global__ __launch_bounds__(64) void plan_chains_kernel(const DeviceOptionRecor
         jb.min_time_steps = 0;
         jb.out_index = slot;
         jb.flags = kJobWriteState;
-        jobs[slot] = jb;
     };
     if (e - b <= 64) {
         // Segments of up to one wavefront: lane i holds option i and the mask

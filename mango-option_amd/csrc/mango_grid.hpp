@@ -110,19 +110,28 @@ __host__ __device__ inline bool make_multi_sinh(GridDesc& g, double x_min, doubl
 // max_d_over_k: the largest D/K of the discrete dividends strictly inside
 // (0, T); the left edge moves to ln(e^x_min - D/K) so the dividend shift stays
 // on the grid, or one unit left when e^x_min <= D/K (grid_spec_types.cpp:56-69).
-__host__ __device__ inline GridDesc estimate_spatial(double spot, double strike, double maturity,
-                                                     double vol, const Accuracy& acc, double max_d_over_k = 0.0) {
+// Domain and point count of estimate_spatial (x0 = ln(S/K) is returned too).
+__host__ __device__ inline void spatial_bounds(double spot, double strike, double maturity, double vol,
+                                               const Accuracy& acc, double max_d_over_k, double& x0,
+                                               double& x_min, double& x_max, int64_t& nx) {
     const double sigma_sqrt_T = vol * sqrt(maturity);
-    const double x0 = log(spot / strike);
-    double x_min = x0 - acc.n_sigma * sigma_sqrt_T;
-    const double x_max = x0 + acc.n_sigma * sigma_sqrt_T;
+    x0 = log(spot / strike);
+    x_min = x0 - acc.n_sigma * sigma_sqrt_T;
+    x_max = x0 + acc.n_sigma * sigma_sqrt_T;
     const double dx_target = vol * sqrt(acc.tol);
-    int64_t nx = static_cast<int64_t>(ceil((x_max - x_min) / dx_target));
+    nx = static_cast<int64_t>(ceil((x_max - x_min) / dx_target));
     if (nx < acc.min_spatial_points) nx = acc.min_spatial_points;
     if (nx > acc.max_spatial_points) nx = acc.max_spatial_points;
     if (nx % 2 == 0) nx++;
     if (max_d_over_k > 0.0 && exp(x_min) > max_d_over_k) x_min = log(exp(x_min) - max_d_over_k);
     else if (max_d_over_k > 0.0) x_min -= 1.0;
+}
+
+__host__ __device__ inline GridDesc estimate_spatial(double spot, double strike, double maturity,
+                                                     double vol, const Accuracy& acc, double max_d_over_k = 0.0) {
+    double x0, x_min, x_max;
+    int64_t nx;
+    spatial_bounds(spot, strike, maturity, vol, acc, max_d_over_k, x0, x_min, x_max, nx);
     double c[2], a[2], w[2];
     int nc = 0;
     c[nc] = x0; a[nc] = acc.alpha; w[nc] = 0.5; ++nc;
