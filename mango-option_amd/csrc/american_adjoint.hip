@@ -18,7 +18,9 @@
 //      from its checkpoint storing u^n and the stage-1 state per step, then
 //      reverse its steps: stage 2, then stage 1 (Rannacher's two Euler
 //      half steps for step 0). Each stage reverse is three cross-lane affine
-//      scans (back-substitution carry, rhs adjoint, denominator adjoint);
+//      scans (the stage's d' again, the back-substitution carry, the rhs
+//      adjoint); the denominator adjoint's sources are accumulated and solved
+//      by one scan per factorization change;
 //   4. the per-node adjoints of the Jacobian bands are contracted with the
 //      stencil weights (L = a d2 + b d1 + c I, a = sigma^2/2,
 //      b = r - q - sigma^2/2, c = -r) and summed over the group:
