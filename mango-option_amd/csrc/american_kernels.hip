@@ -870,9 +870,10 @@ void set_variant_rules(const int64_t* max_n, const SolveVariant* v, int n) {
 SolveVariant pick_variant(int64_t n_space) {
     for (int i = 0; i < g_n_rules; ++i)
         if (n_space <= g_rules[i].max_n) return g_rules[i].v;
+    if (n_space <= 64) return {16, 4};
     if (n_space <= 112) return {16, 7};
-    if (n_space <= 128) return {32, 4};
-    if (n_space <= 256) return {64, 4};
+    if (n_space <= 128) return {16, 8};
+    if (n_space <= 256) return {32, 8};
     return {64, 7};
 }
 

@@ -346,13 +346,24 @@ MangoStatus run_jobs(std::vector<PdeJob>& jobs, size_t n_out, Greeks* greeks, in
     sorted.reserve(jobs.size());
     std::vector<std::pair<SolveVariant, std::pair<size_t, size_t>>> ranges;
     {
+        // throughput variant per job, except where its bin has no more jobs
+        // than the chip has SIMDs: such a bin runs one option per wavefront
+        // (the latency variant), as a whole batch that small does
         std::vector<SolveVariant> var(jobs.size()), all;
+        std::vector<std::pair<SolveVariant, size_t>> bin_count;
+        auto same = [](const SolveVariant& a, const SolveVariant& b) { return a.lanes == b.lanes && a.nodes == b.nodes; };
         for (size_t i = 0; i < jobs.size(); ++i) {
             if (jobs[i].ext != 0) continue;
-            const int64_t nsp = std::max<int64_t>(jobs[i].grid.n_points, 3);
-            var[i] = n_plain <= static_cast<size_t>(kLatencyJobs) ? pick_latency_variant(nsp) : pick_variant(nsp);
-            if (std::none_of(all.begin(), all.end(), [&](const SolveVariant& a) {
-                    return a.lanes == var[i].lanes && a.nodes == var[i].nodes; }))
+            var[i] = pick_variant(std::max<int64_t>(jobs[i].grid.n_points, 3));
+            auto it = std::find_if(bin_count.begin(), bin_count.end(), [&](const auto& b) { return same(b.first, var[i]); });
+            if (it == bin_count.end()) bin_count.push_back({var[i], 1}); else ++it->second;
+        }
+        for (size_t i = 0; i < jobs.size(); ++i) {
+            if (jobs[i].ext != 0) continue;
+            const auto it = std::find_if(bin_count.begin(), bin_count.end(), [&](const auto& b) { return same(b.first, var[i]); });
+            if (n_plain <= static_cast<size_t>(kLatencyJobs) || it->second <= static_cast<size_t>(kLatencyJobs))
+                var[i] = pick_latency_variant(std::max<int64_t>(jobs[i].grid.n_points, 3));
+            if (std::none_of(all.begin(), all.end(), [&](const SolveVariant& a) { return same(a, var[i]); }))
                 all.push_back(var[i]);
         }
         for (const auto& v : all) {
