@@ -823,7 +823,7 @@ int launch_variant(const PdeJob* d_jobs, int n_jobs, const int32_t* d_n_jobs, Gr
 }  // namespace
 
 // Compiled (lanes, nodes-per-lane) variants. Each covers n <= lanes * nodes.
-#define MANGO_VARIANTS(X) X(16, 4) X(16, 7) X(16, 8) X(32, 2) X(32, 4) X(32, 8) X(64, 2) X(64, 4) X(64, 7)
+#define MANGO_VARIANTS(X) X(16, 4) X(16, 7) X(16, 8) X(32, 2) X(32, 4) X(32, 8) X(64, 2) X(64, 4) X(64, 5) X(64, 7)
 
 bool variant_supported(const SolveVariant& v) {
 #define MANGO_IS(K, M) if (v.lanes == K && v.nodes == M) return true;
@@ -874,6 +874,7 @@ SolveVariant pick_variant(int64_t n_space) {
     if (n_space <= 112) return {16, 7};
     if (n_space <= 128) return {16, 8};
     if (n_space <= 256) return {32, 8};
+    if (n_space <= 320) return {64, 5};
     return {64, 7};
 }
 
@@ -883,6 +884,7 @@ SolveVariant pick_latency_variant(int64_t n_space) {
     if (g_n_rules > 0) return pick_variant(n_space);
     if (n_space <= 128) return {64, 2};
     if (n_space <= 256) return {64, 4};
+    if (n_space <= 320) return {64, 5};
     return {64, 7};
 }
 
